@@ -1,0 +1,242 @@
+#!/usr/bin/env python3
+"""Throughput benchmark: large single transposes on one MI355X.
+
+Workload (named large cases): seven ~256 MiB transposes, float32
+and float64, covering the Tiled, TiledCopy and Packed/PackedSplit paths:
+
+    f32 (8192,8192)/(1,0)           f32 (1024,256,256)/(0,2,1)
+    f32 (2,8,16,32,64,128)/(5,3,1,0,2,4)
+    f64 (4096,8192)/(1,0)           f64 (512,256,256)/(0,2,1)
+    f64 (4,4,8,32,32,256)/(2,5,0,4,1,3)
+    f64 (5,97,211,331)/(3,1,0,2)
+
+A step is one pass over the seven cases with inputs resident in HBM. The
+metric is 2*bytes/t summed over the step (the reference's librettTimer count,
+src/Timer.cpp:151). The line also carries a same-run hipMemcpyAsync D2D
+ceiling, the roofline of the dominant kernel (HIP events on the plan's
+stream) and a CPU baseline (the OpenMP host permutation on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CASES = [
+    ((8192, 8192), (1, 0), 4),
+    ((1024, 256, 256), (0, 2, 1), 4),
+    ((2, 8, 16, 32, 64, 128), (5, 3, 1, 0, 2, 4), 4),
+    ((4096, 8192), (1, 0), 8),
+    ((512, 256, 256), (0, 2, 1), 8),
+    ((4, 4, 8, 32, 32, 256), (2, 5, 0, 4, 1, 3), 8),
+    ((5, 97, 211, 331), (3, 1, 0, 2), 8),
+]
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--measure", action="store_true", help="choose plans with librettPlanMeasure")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X and has no "
+              "multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import librett_amd as L
+    from librett_amd import data, hip
+
+    hip.check(hip.runtime().hipSetDevice(0), "hipSetDevice")
+    stream = hip.stream_create()
+
+    # One random source buffer, uploaded as the input of every case.
+    max_bytes = max(int(np.prod(d)) * e for d, _, e in CASES)
+    src = data.uniform(max_bytes // 8 + 1, 8, 20261202).view(np.uint8)[:max_bytes]
+    cases = []
+    for dim, perm, elem in CASES:
+        vol = int(np.prod(dim))
+        nbytes = vol * elem
+        a = hip.DeviceBuffer(nbytes)
+        b = hip.DeviceBuffer(nbytes)
+        hip.htod(a.ptr, src[:nbytes])
+        hip.memset(b.ptr, 0, nbytes)
+        t0 = time.perf_counter()
+        if args.measure:
+            r, h = L.librettPlanMeasure(len(dim), list(dim), list(perm), elem, stream, a.ptr, b.ptr)
+        else:
+            r, h = L.librettPlan(len(dim), list(dim), list(perm), elem, stream)
+        plan_s = time.perf_counter() - t0
+        if r != L.LIBRETT_SUCCESS:
+            raise SystemExit("plan failed for %s %s: %d" % (dim, perm, r))
+        info = L.plan_info(h)
+        cases.append(dict(dim=dim, perm=perm, elem=elem, vol=vol, bytes=2 * nbytes, a=a, b=b, h=h, info=info,
+                          plan_s=plan_s))
+        log("plan %-28s %-20s %d B -> %s (%.1f ms)" % (dim, perm, elem, info["desc"], plan_s * 1e3))
+
+    def run_step(events=None, k=0):
+        for i, c in enumerate(cases):
+            if events is not None:
+                events[i][k][0].record(stream)
+            rc = L.librettExecute(c["h"], c["a"].ptr, c["b"].ptr)
+            if rc != L.LIBRETT_SUCCESS:
+                raise SystemExit("execute failed: %d" % rc)
+            if events is not None:
+                events[i][k][1].record(stream)
+
+    for _ in range(args.warmup):
+        run_step()
+    hip.stream_sync(stream)
+    log("warmup done (%d steps)" % args.warmup)
+
+    events = [[(hip.Event(), hip.Event()) for _ in range(args.steps)] for _ in cases]
+    hip.device_sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        run_step(events, k)
+    hip.stream_sync(stream)
+    hip.device_sync()
+    wall = time.perf_counter() - t0
+    log("timed %d steps in %.3f s" % (args.steps, wall))
+
+    step_bytes = sum(c["bytes"] for c in cases)
+    value = step_bytes * args.steps / wall / 1e9
+    per_case = []
+    by_kernel = {}
+    for i, c in enumerate(cases):
+        ts = sorted(events[i][k][1].elapsed_ms(events[i][k][0]) for k in range(args.steps))
+        avg = sum(ts) / len(ts)
+        med = ts[len(ts) // 2]
+        gbs = c["bytes"] / (avg * 1e-3) / 1e9
+        per_case.append(dict(dim=list(c["dim"]), perm=list(c["perm"]), elem=c["elem"], plan=c["info"]["desc"],
+                             method=c["info"]["method_name"], avg_ms=round(avg, 4), median_ms=round(med, 4),
+                             best_ms=round(ts[0], 4), gbs=round(gbs, 1),
+                             predicted_ms=round(c["info"]["predicted_s"] * 1e3, 4)))
+        kname = c["info"]["method_name"] + ":" + c["info"]["desc"].split(" grid")[0]
+        kb = by_kernel.setdefault(kname, [0.0, 0, 0])
+        kb[0] += avg * args.steps
+        kb[1] += c["bytes"] * args.steps
+        kb[2] += args.steps
+    dom_name, (dom_ms, dom_bytes, dom_launches) = max(by_kernel.items(), key=lambda kv: kv[1][0])
+    dom_gbs = dom_bytes / (dom_ms * 1e-3) / 1e9
+
+    # Same-run D2D ceiling: hipMemcpyAsync of the largest case's input.
+    big = max(cases, key=lambda c: c["bytes"])
+    nb = big["bytes"] // 2
+    e0, e1 = hip.Event(), hip.Event()
+    for _ in range(3):
+        hip.memcpy_d2d_async(big["b"].ptr, big["a"].ptr, nb, stream)
+    d2d = []
+    for _ in range(10):
+        e0.record(stream)
+        hip.memcpy_d2d_async(big["b"].ptr, big["a"].ptr, nb, stream)
+        e1.record(stream)
+        e1.synchronize()
+        d2d.append(e1.elapsed_ms(e0))
+    d2d.sort()
+    d2d_gbs = 2 * nb / (d2d[len(d2d) // 2] * 1e-3) / 1e9
+
+    for c in per_case:
+        c["frac_d2d"] = round(c["gbs"] / d2d_gbs, 3)
+        log("case %-28s %-20s %2d B %-11s %8.1f GB/s (%5.1f%% of D2D, %5.1f%% of 8 TB/s) model %.3f ms, run %.3f ms"
+            % (tuple(c["dim"]), tuple(c["perm"]), c["elem"], c["method"], c["gbs"], 100 * c["frac_d2d"],
+               100 * c["gbs"] / HBM_PEAK_GBS, c["predicted_ms"], c["avg_ms"]))
+    log("D2D hipMemcpyAsync %.1f GB/s; aggregate %.1f GB/s (%.1f%% of D2D)" % (d2d_gbs, value, 100 * value / d2d_gbs))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_reps)
+
+    for c in cases:
+        L.librettDestroy(c["h"])
+        c["a"].free()
+        c["b"].free()
+    hip.stream_destroy(stream)
+
+    line = {
+        "metric": "transpose GB/s (2*bytes/t) vs HBM peak",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 (opaque 4/8-byte moves)",
+        "data": "synthetic (seeded uniform [-1,1))",
+        "config": {
+            "workload": "large single transposes ~256 MiB each (7 named cases, f32 and f64)",
+            "cases": len(cases),
+            "bytes_per_step": step_bytes,
+            "plan_selection": "librettPlanMeasure" if args.measure else "gfx950 model (librettPlan)",
+        },
+        "d2d_copy_gbs": round(d2d_gbs, 1),
+        "frac_of_d2d": round(value / d2d_gbs, 3),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(dom_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "launches": dom_launches,
+            "avg_launch_ms": round(dom_ms / dom_launches, 4),
+        },
+        "cpu_baseline": cpu,
+        "cases": per_case,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(reps):
+    """The oracle's OpenMP permutation on the host, over the same seven cases
+    at full size (the bounded sample is one median-of-`reps` run per case)."""
+    import oracle
+    from librett_amd import data
+    nthreads = min(oracle.max_threads(), int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or oracle.max_threads())
+    total_bytes = 0
+    total_s = 0.0
+    t_start = time.perf_counter()
+    max_bytes = max(int(np.prod(d)) * e for d, _, e in CASES)
+    src = data.uniform(max_bytes // 8 + 1, 8, 5).view(np.uint8)
+    for dim, perm, elem in CASES:
+        vol = int(np.prod(dim))
+        x = src[:vol * elem].view(oracle.ELEM_DTYPES[elem])
+        oracle.transpose(list(dim), list(perm), elem, x, variant="omp", nthreads=nthreads)  # warm-up
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            oracle.transpose(list(dim), list(perm), elem, x, variant="omp", nthreads=nthreads)
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        total_s += ts[len(ts) // 2]
+        total_bytes += 2 * vol * elem
+        log("cpu %-28s %.3f s" % (dim, ts[len(ts) // 2]))
+    log("cpu baseline took %.1f s wall" % (time.perf_counter() - t_start))
+    return {
+        "value": round(total_bytes / total_s / 1e9, 2),
+        "unit": "GB/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": "the 7 named cases at full size, median of %d OpenMP runs each after one warm-up" % reps,
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,119 @@
+/*
+ * librett_c.h -- C-ABI twin of librett.h for foreign-function callers
+ * (Python ctypes, Go cgo, Java JNI ...): plain pointers and sizes, the stream
+ * passed by value as an opaque pointer, no C++ types.
+ *
+ * Each librett_c_* entry point below names the reference interface it
+ * replaces. The entry points marked "extension" have no reference
+ * counterpart; they expose plan introspection and host-side planning for
+ * tests, benchmarks and tuning tools.
+ */
+#ifndef LIBRETT_C_H
+#define LIBRETT_C_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same values as librettResult (reference src/librett.h:43-50). */
+enum {
+  LIBRETT_C_SUCCESS = 0,
+  LIBRETT_C_INVALID_PLAN = 1,
+  LIBRETT_C_INVALID_PARAMETER = 2,
+  LIBRETT_C_INVALID_DEVICE = 3,
+  LIBRETT_C_INTERNAL_ERROR = 4,
+  LIBRETT_C_UNDEFINED_ERROR = 5
+};
+
+/* Method numbers, as the reference's enum (src/plan.h:54-56). */
+enum {
+  LIBRETT_C_TRIVIAL = 1,
+  LIBRETT_C_PACKED = 2,
+  LIBRETT_C_PACKED_SPLIT = 3,
+  LIBRETT_C_TILED = 4,
+  LIBRETT_C_TILED_COPY = 5
+};
+
+/* Replaces librettInitialize / librettFinalize (src/librett.h:57,62). */
+void librett_c_initialize(void);
+void librett_c_finalize(void);
+
+/* Replaces librettPlan (src/librett.h:78); `stream` is a hipStream_t. */
+int librett_c_plan(unsigned int* handle, int rank, const int* dim, const int* permutation, size_t sizeofType,
+                   void* stream);
+
+/* Replaces librettPlanMeasure (src/librett.h:96-97). */
+int librett_c_plan_measure(unsigned int* handle, int rank, const int* dim, const int* permutation,
+                           size_t sizeofType, void* stream, void* idata, void* odata);
+
+/* Replaces librettDestroy (src/librett.h:108). */
+int librett_c_destroy(unsigned int handle);
+
+/* Replaces librettExecute (src/librett.h:121). */
+int librett_c_execute(unsigned int handle, const void* idata, void* odata);
+
+/* Description of a plan or of a candidate plan. */
+typedef struct librett_c_plan_info {
+  int method;        /* LIBRETT_C_TRIVIAL ... LIBRETT_C_TILED_COPY */
+  int elem;          /* bytes per element */
+  int tile_x, tile_y, vec_x, vec_y; /* Tiled */
+  int word, copy_tx;                /* TiledCopy */
+  int nreg, num_split, split_rank;  /* Packed / PackedSplit */
+  int size_mm, size_mk, vol_mmk, vol_mbar;
+  int grid_x, grid_y, block, lds_bytes, active_blocks;
+  double predicted_s;    /* gfx950 model estimate */
+  double measured_s;     /* last librett_c_plan_measure time, or -1 */
+  double read_sectors, write_sectors, partial_writes;
+  long long bytes;       /* algorithmic bytes, 2 * vol * elem */
+  char desc[192];
+} librett_c_plan_info;
+
+/* Device description for host-only planning (extension). */
+typedef struct librett_c_device {
+  int cu_count;
+  int lds_per_block;
+  int lds_per_cu;
+  int max_threads_per_block;
+  int max_waves_per_cu;
+  double hbm_bytes_per_s;
+} librett_c_device;
+
+/* Extension: describes a stored plan. */
+int librett_c_plan_info_get(unsigned int handle, librett_c_plan_info* info);
+
+/* Extension: number of candidates and their descriptions for a problem on
+ * the current device (at most `max` are written). */
+int librett_c_candidates(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                         librett_c_plan_info* infos, int max, int* count);
+
+/* Extension: plan that uses candidate `index` of librett_c_candidates. */
+int librett_c_plan_candidate(unsigned int* handle, int rank, const int* dim, const int* permutation,
+                             size_t sizeofType, void* stream, int index);
+
+/* Extension: host-only candidate list for a described device (no GPU). */
+int librett_c_host_candidates(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                              const librett_c_device* dev, librett_c_plan_info* infos, int max, int* count);
+
+/* Extension: runs candidate `index` of librett_c_host_candidates on the host
+ * (the kernels' per-thread code with barrier order kept); host buffers.
+ * The replay runs between guard bands and returns LIBRETT_C_UNDEFINED_ERROR
+ * if any byte outside the output tensor was written. */
+int librett_c_host_replay(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                          const librett_c_device* dev, int index, const void* idata, void* odata);
+
+/* Extension: host-only rank reduction (reference reduceRanks,
+ * src/plan.cpp:64-127) after size-1 dims are dropped. Writes the reduced
+ * rank, dims and permutation. */
+int librett_c_reduce_ranks(int rank, const int* dim, const int* permutation, int* red_rank, int* red_dim,
+                           int* red_perm);
+
+/* Extension: library version string. */
+const char* librett_c_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIBRETT_C_H */

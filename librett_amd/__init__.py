@@ -1,0 +1,216 @@
+"""Python binding of the gfx950 transpose library (``librett.so``).
+
+The library itself is C++/HIP; this module binds its C-ABI twin
+(``include/librett_c.h``) with ctypes so tests and benchmarks can drive it.
+Names and result codes follow the reference API (``src/librett.h:40-121``):
+``librettPlan``, ``librettPlanMeasure``, ``librettExecute``,
+``librettDestroy``, ``librettInitialize``, ``librettFinalize``.
+
+There is no fallback: if ``librett.so`` is missing, importing the binding
+raises, so nothing can silently run a host path instead of the HIP kernels.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librett.so")
+
+LIBRETT_SUCCESS = 0
+LIBRETT_INVALID_PLAN = 1
+LIBRETT_INVALID_PARAMETER = 2
+LIBRETT_INVALID_DEVICE = 3
+LIBRETT_INTERNAL_ERROR = 4
+LIBRETT_UNDEFINED_ERROR = 5
+
+TRIVIAL, PACKED, PACKED_SPLIT, TILED, TILED_COPY = 1, 2, 3, 4, 5
+METHOD_NAMES = {1: "Trivial", 2: "Packed", 3: "PackedSplit", 4: "Tiled", 5: "TiledCopy"}
+
+_LIB = None
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int), ("elem", ctypes.c_int),
+        ("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int), ("vec_x", ctypes.c_int), ("vec_y", ctypes.c_int),
+        ("word", ctypes.c_int), ("copy_tx", ctypes.c_int),
+        ("nreg", ctypes.c_int), ("num_split", ctypes.c_int), ("split_rank", ctypes.c_int),
+        ("size_mm", ctypes.c_int), ("size_mk", ctypes.c_int), ("vol_mmk", ctypes.c_int), ("vol_mbar", ctypes.c_int),
+        ("grid_x", ctypes.c_int), ("grid_y", ctypes.c_int), ("block", ctypes.c_int), ("lds_bytes", ctypes.c_int),
+        ("active_blocks", ctypes.c_int),
+        ("predicted_s", ctypes.c_double), ("measured_s", ctypes.c_double),
+        ("read_sectors", ctypes.c_double), ("write_sectors", ctypes.c_double), ("partial_writes", ctypes.c_double),
+        ("bytes", ctypes.c_longlong),
+        ("desc", ctypes.c_char * 192),
+    ]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["desc"] = self.desc.decode()
+        d["method_name"] = METHOD_NAMES.get(self.method, "?")
+        return d
+
+
+class DeviceDesc(ctypes.Structure):
+    _fields_ = [
+        ("cu_count", ctypes.c_int), ("lds_per_block", ctypes.c_int), ("lds_per_cu", ctypes.c_int),
+        ("max_threads_per_block", ctypes.c_int), ("max_waves_per_cu", ctypes.c_int),
+        ("hbm_bytes_per_s", ctypes.c_double),
+    ]
+
+
+def mi355x_desc():
+    """Published MI355X figures for host-only planning."""
+    return DeviceDesc(256, 160 * 1024, 160 * 1024, 1024, 32, 8.0e12)
+
+
+EXPORTED = [
+    "librett_c_initialize", "librett_c_finalize", "librett_c_plan", "librett_c_plan_measure",
+    "librett_c_destroy", "librett_c_execute", "librett_c_plan_info_get", "librett_c_candidates",
+    "librett_c_plan_candidate", "librett_c_host_candidates", "librett_c_host_replay",
+    "librett_c_reduce_ranks", "librett_c_version",
+]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("librett.so is not built (run __graft_entry__.build() or make -C librett_amd)")
+        from . import hip
+        hip.runtime()  # the HIP runtime the library links, loaded first and globally
+        L = ctypes.CDLL(LIB_PATH)
+        ip = ctypes.POINTER(ctypes.c_int)
+        up = ctypes.POINTER(ctypes.c_uint)
+        vp = ctypes.c_void_p
+        L.librett_c_initialize.argtypes = []
+        L.librett_c_finalize.argtypes = []
+        L.librett_c_plan.argtypes = [up, ctypes.c_int, ip, ip, ctypes.c_size_t, vp]
+        L.librett_c_plan_measure.argtypes = [up, ctypes.c_int, ip, ip, ctypes.c_size_t, vp, vp, vp]
+        L.librett_c_destroy.argtypes = [ctypes.c_uint]
+        L.librett_c_execute.argtypes = [ctypes.c_uint, vp, vp]
+        L.librett_c_plan_info_get.argtypes = [ctypes.c_uint, ctypes.POINTER(PlanInfo)]
+        L.librett_c_candidates.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, ctypes.POINTER(PlanInfo),
+                                           ctypes.c_int, ip]
+        L.librett_c_plan_candidate.argtypes = [up, ctypes.c_int, ip, ip, ctypes.c_size_t, vp, ctypes.c_int]
+        L.librett_c_host_candidates.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, ctypes.POINTER(DeviceDesc),
+                                                ctypes.POINTER(PlanInfo), ctypes.c_int, ip]
+        L.librett_c_host_replay.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, ctypes.POINTER(DeviceDesc),
+                                            ctypes.c_int, vp, vp]
+        L.librett_c_reduce_ranks.argtypes = [ctypes.c_int, ip, ip, ip, ip, ip]
+        L.librett_c_version.restype = ctypes.c_char_p
+        for name in EXPORTED:
+            f = getattr(L, name)
+            if name != "librett_c_version" and f.restype is not ctypes.c_char_p:
+                f.restype = ctypes.c_int
+        L.librett_c_initialize.restype = None
+        L.librett_c_finalize.restype = None
+        _LIB = L
+    return _LIB
+
+
+def _iarr(v):
+    return (ctypes.c_int * max(1, len(v)))(*[int(x) for x in v])
+
+
+# ---------------------------------------------------------------------------
+# Reference-named entry points
+# ---------------------------------------------------------------------------
+
+def librettInitialize():
+    lib().librett_c_initialize()
+
+
+def librettFinalize():
+    lib().librett_c_finalize()
+
+
+def librettPlan(rank, dim, permutation, sizeofType, stream):
+    """Returns (result, handle)."""
+    h = ctypes.c_uint(0)
+    r = lib().librett_c_plan(ctypes.byref(h), int(rank), _iarr(dim), _iarr(permutation), int(sizeofType),
+                             ctypes.c_void_p(stream))
+    return r, h.value
+
+
+def librettPlanMeasure(rank, dim, permutation, sizeofType, stream, idata, odata):
+    """Returns (result, handle)."""
+    h = ctypes.c_uint(0)
+    r = lib().librett_c_plan_measure(ctypes.byref(h), int(rank), _iarr(dim), _iarr(permutation),
+                                     int(sizeofType), ctypes.c_void_p(stream), ctypes.c_void_p(idata),
+                                     ctypes.c_void_p(odata))
+    return r, h.value
+
+
+def librettExecute(handle, idata, odata):
+    return lib().librett_c_execute(ctypes.c_uint(handle), ctypes.c_void_p(idata), ctypes.c_void_p(odata))
+
+
+def librettDestroy(handle):
+    return lib().librett_c_destroy(ctypes.c_uint(handle))
+
+
+# ---------------------------------------------------------------------------
+# Extensions
+# ---------------------------------------------------------------------------
+
+def plan_candidate(rank, dim, permutation, sizeofType, stream, index):
+    h = ctypes.c_uint(0)
+    r = lib().librett_c_plan_candidate(ctypes.byref(h), int(rank), _iarr(dim), _iarr(permutation),
+                                       int(sizeofType), ctypes.c_void_p(stream), int(index))
+    return r, h.value
+
+
+def plan_info(handle):
+    info = PlanInfo()
+    r = lib().librett_c_plan_info_get(ctypes.c_uint(handle), ctypes.byref(info))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("plan_info failed: %d" % r)
+    return info.as_dict()
+
+
+def candidates(dim, permutation, sizeofType, max_count=512):
+    infos = (PlanInfo * max_count)()
+    n = ctypes.c_int(0)
+    r = lib().librett_c_candidates(len(dim), _iarr(dim), _iarr(permutation), int(sizeofType), infos, max_count,
+                                   ctypes.byref(n))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("candidates failed: %d" % r)
+    return [infos[i].as_dict() for i in range(min(n.value, max_count))]
+
+
+def host_candidates(dim, permutation, sizeofType, device=None, max_count=512):
+    dev = device or mi355x_desc()
+    infos = (PlanInfo * max_count)()
+    n = ctypes.c_int(0)
+    r = lib().librett_c_host_candidates(len(dim), _iarr(dim), _iarr(permutation), int(sizeofType),
+                                        ctypes.byref(dev), infos, max_count, ctypes.byref(n))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("host_candidates failed: %d" % r)
+    return [infos[i].as_dict() for i in range(min(n.value, max_count))]
+
+
+def host_replay(dim, permutation, sizeofType, index, inp, device=None):
+    """Runs candidate `index` on the host; `inp` is a numpy array."""
+    import numpy as np
+    dev = device or mi355x_desc()
+    inp = np.ascontiguousarray(inp)
+    out = np.zeros_like(inp)
+    r = lib().librett_c_host_replay(len(dim), _iarr(dim), _iarr(permutation), int(sizeofType), ctypes.byref(dev),
+                                    int(index), ctypes.c_void_p(inp.ctypes.data), ctypes.c_void_p(out.ctypes.data))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("host_replay failed: %d" % r)
+    return out
+
+
+def reduce_ranks(dim, permutation):
+    rr = ctypes.c_int(0)
+    rd = (ctypes.c_int * 64)()
+    rp = (ctypes.c_int * 64)()
+    r = lib().librett_c_reduce_ranks(len(dim), _iarr(dim), _iarr(permutation), ctypes.byref(rr), rd, rp)
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("reduce_ranks failed: %d" % r)
+    return [rd[i] for i in range(rr.value)], [rp[i] for i in range(rr.value)]
+
+
+def version():
+    return lib().librett_c_version().decode()

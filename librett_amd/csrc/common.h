@@ -1,0 +1,152 @@
+// Shared host/device definitions for the gfx950 transpose library.
+//
+// Everything here is plain data passed by value in kernel arguments, so a
+// plan needs no device-side tables (the reference uploads TensorConv tables
+// with hipMalloc + hipMemcpyAsync in librettPlan_t::activate,
+// src/plan.cpp:1648-1676; here the same information travels in the kernel
+// argument segment and is read with scalar loads).
+#pragma once
+
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+namespace lrt {
+
+// Largest rank after size-1 dims are removed. Every remaining dim is >= 2 and
+// the volume is < 2^31, so at most 30 dims remain.
+constexpr int kMaxRank = 32;
+// Largest number of dims staged on chip by the Packed kernels.
+constexpr int kMaxMmk = 12;
+
+// Method numbering follows the reference enum (src/plan.h:54-56).
+enum Method : int {
+  kUnknown = 0,
+  kTrivial = 1,
+  kPacked = 2,
+  kPackedSplit = 3,
+  kTiled = 4,
+  kTiledCopy = 5,
+  kNumMethods = 6
+};
+
+// Division by an invariant 32-bit divisor for dividends below 2^31
+// (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, round-up variant): with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1, n / d = (umulhi(n, m) + n) >> l.
+// umulhi(n, m) <= n, so the sum stays below 2^32 while n < 2^31.
+struct FastDiv {
+  uint32_t mul;
+  uint32_t shr;
+  uint32_t d;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  uint32_t l = 0;
+  while ((1ull << l) < d) l++;
+  uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+  f.mul = (uint32_t)m;
+  f.shr = l;
+  f.d = d;
+  return f;
+}
+
+__host__ __device__ inline uint32_t umulhi32(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umulhi(a, b);
+#else
+  return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+#endif
+}
+
+__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv& f) {
+  return (umulhi32(n, f.mul) + n) >> f.shr;
+}
+
+// Outer ("Mbar") index space: dims that are neither staged on chip nor part
+// of the tile plane, in input order. Index k of this space has digits
+// (k % d0, (k / d0) % d1, ...); digit r moves the input by ct_in[r] and the
+// output by ct_out[r] elements (the reference keeps the same information as
+// TensorConvInOut entries, src/plan.cpp:935-947).
+struct OuterMap {
+  int n;
+  FastDiv div[kMaxRank];
+  int ct_in[kMaxRank];
+  int ct_out[kMaxRank];
+};
+
+__host__ __device__ inline void outer_offsets(const OuterMap& m, uint32_t k,
+                                              int& off_in, int& off_out) {
+  int oi = 0, oo = 0;
+  for (int r = 0; r < m.n; r++) {
+    uint32_t q = fdiv(k, m.div[r]);
+    int digit = (int)(k - q * m.div[r].d);
+    oi += digit * m.ct_in[r];
+    oo += digit * m.ct_out[r];
+    k = q;
+  }
+  off_in = oi;
+  off_out = oo;
+}
+
+// Tiled: 2-D tile plane of (x = input dim 0, y = the input dim that becomes
+// output dim 0). Reference counterpart: transposeTiled, src/kernel.cpp:48-170.
+struct TiledArgs {
+  int X, Y;          // plane extents in elements
+  int in_ys;         // input stride of y
+  int out_xs;        // output stride of x
+  int ntx;           // tiles along x
+  int ntiles;        // tiles per plane
+  FastDiv ntx_div;
+  FastDiv ntiles_div;
+  int items;         // ntiles * volMbar
+  OuterMap outer;
+};
+
+// TiledCopy: rows that stay contiguous in input and output (perm[0] == 0),
+// moved as W-byte words. Plane = (x = words of dim 0, y = the input dim that
+// becomes output dim 1). Reference counterpart: transposeTiledCopy,
+// src/kernel.cpp:453-545.
+struct CopyArgs {
+  int X, Y;          // plane extents (x in words)
+  int in_ys;         // input stride of y in words
+  int out_ys;        // output stride of y in words
+  int ntx;
+  int ntiles;
+  FastDiv ntx_div;
+  FastDiv ntiles_div;
+  int items;
+  OuterMap outer;    // strides in words
+};
+
+// One on-chip ("Mmk") sub-tensor shape. The LDS holds it in input order, so
+// LDS position p is also its input-order linear index. Input-order digits of
+// p give the global input offset; output-order digits of the output-ordered
+// index u give the global output offset and the LDS position to read.
+// Reference counterpart: TensorConvInOut Mmk and TensorConv Msh tables,
+// src/plan.cpp:1009-1058 (Packed) and :953-1007 (PackedSplit).
+struct StagedShape {
+  int vol;
+  int n;
+  FastDiv din[kMaxMmk];
+  int ct_in[kMaxMmk];
+  FastDiv dout[kMaxMmk];
+  int ct_out[kMaxMmk];
+  int ct_sh[kMaxMmk];
+};
+
+// Packed and PackedSplit share one kernel: Packed is a split count of 1.
+// Reference counterparts: transposePacked (src/kernel.cpp:175-296) and
+// transposePackedSplit (src/kernel.cpp:304-444).
+struct PackedArgs {
+  StagedShape shape[2];  // [0]: chunk of floor(splitDim/numSplit), [1]: +1
+  int num_split;
+  int split_dim;
+  int split_in_stride;   // input stride of the split dim
+  int split_out_stride;  // output stride of the split dim
+  int vol_outer;
+  int outer_per_block;   // contiguous outer indices per block row
+  OuterMap outer;
+};
+
+}  // namespace lrt

@@ -1,0 +1,28 @@
+// Launch, replay and occupancy entry points of the kernel table (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "planner.h"
+
+namespace lrt {
+
+// Enqueues the candidate's kernel (or the D2D copy for Trivial) on `stream`.
+hipError_t launch_candidate(const Candidate& c, const void* in, void* out, hipStream_t stream);
+
+// Runs the candidate's per-thread code on the host (CPU tests). Returns
+// false if the candidate names a kernel shape that is not instantiated.
+bool replay_candidate(const Candidate& c, const void* in, void* out);
+
+// Runtime occupancy (active blocks per CU) of the candidate's kernel.
+int query_occupancy(const Candidate& c);
+
+// Sets per-kernel attributes a launch needs (dynamic LDS size).
+int prepare_kernel(const Candidate& c);
+
+bool tiled_shape_available(int elem, int tx, int ty, int vx, int vy);
+bool copy_shape_available(int word, int tx);
+int packed_max_nreg(int elem);
+bool packed_nreg_available(int elem, int nreg);
+
+}  // namespace lrt

@@ -1,0 +1,319 @@
+// gfx950 transpose kernels, their launch table, runtime occupancy queries and
+// the host replay used by the CPU tests.
+//
+// Reference counterpart of the dispatch: librettKernel, src/kernel.cpp:1008-1215
+// (method x element type x register-count switch, src/calls.h:1-92).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "kernel_ops.h"
+#include "kernels.h"
+
+namespace lrt {
+
+// ---------------------------------------------------------------------------
+// Device kernels
+// ---------------------------------------------------------------------------
+
+template <int B, int TX, int TY, int VX, int VY>
+__global__ __launch_bounds__(256) void k_tiled(TiledArgs a, const void* __restrict__ in,
+                                               void* __restrict__ out) {
+  using Ops = TiledOps<B, TX, TY, VX, VY>;
+  using E = typename Ops::E;
+  __shared__ E lds[Ops::LDS_ELEMS];
+  const E* src = static_cast<const E*>(in);
+  E* dst = static_cast<E*>(out);
+  for (uint32_t w = blockIdx.x; w < (uint32_t)a.items; w += gridDim.x) {
+    const TileCtx c = Ops::item(a, w);
+    Ops::load(a, c, threadIdx.x, src, lds);
+    __syncthreads();
+    Ops::store(a, c, threadIdx.x, dst, lds);
+    __syncthreads();
+  }
+}
+
+template <int W, int TX, int NP>
+__global__ __launch_bounds__(256) void k_copy(CopyArgs a, const void* __restrict__ in,
+                                              void* __restrict__ out) {
+  using Ops = CopyOps<W, TX, NP>;
+  using V = typename Ops::V;
+  const V* src = static_cast<const V*>(in);
+  V* dst = static_cast<V*>(out);
+  for (uint32_t w = blockIdx.x; w < (uint32_t)a.items; w += gridDim.x) {
+    const TileCtx c = Ops::item(a, w);
+    Ops::run(a, c, threadIdx.x, src, dst);
+  }
+}
+
+template <int B, int NREG>
+__global__ __launch_bounds__(1024) void k_packed(PackedArgs a, const void* __restrict__ in,
+                                                 void* __restrict__ out) {
+  using Ops = PackedOps<B, NREG>;
+  using E = typename Ops::E;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  E* lds = reinterpret_cast<E*>(smem);
+  const E* src = static_cast<const E*>(in);
+  E* dst = static_cast<E*>(out);
+  const int nt = blockDim.x;
+  const int t = threadIdx.x;
+  typename Ops::Pre p;
+  Ops::prepare(a, blockIdx.x, t, nt, p);
+  const int o0 = blockIdx.y * a.outer_per_block;
+  const int o1 = min(a.vol_outer, o0 + a.outer_per_block);
+  for (int o = o0; o < o1; o++) {
+    int off_in, off_out;
+    outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
+    Ops::load(p, off_in, t, nt, src, lds);
+    __syncthreads();
+    Ops::store(p, off_out, t, nt, dst, lds);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host replays: the same per-thread code, barrier order kept by running every
+// thread's load() before any thread's store().
+// ---------------------------------------------------------------------------
+
+template <int B, int TX, int TY, int VX, int VY>
+static void replay_tiled(const Candidate& c, const void* in, void* out) {
+  using Ops = TiledOps<B, TX, TY, VX, VY>;
+  using E = typename Ops::E;
+  std::vector<E> lds(Ops::LDS_ELEMS);
+  const TiledArgs& a = c.tiled;
+  for (uint32_t w = 0; w < (uint32_t)a.items; w++) {
+    const TileCtx ctx = Ops::item(a, w);
+    for (int t = 0; t < Ops::NT; t++) Ops::load(a, ctx, t, static_cast<const E*>(in), lds.data());
+    for (int t = 0; t < Ops::NT; t++) Ops::store(a, ctx, t, static_cast<E*>(out), lds.data());
+  }
+}
+
+template <int W, int TX, int NP>
+static void replay_copy(const Candidate& c, const void* in, void* out) {
+  using Ops = CopyOps<W, TX, NP>;
+  using V = typename Ops::V;
+  const CopyArgs& a = c.copy;
+  for (uint32_t w = 0; w < (uint32_t)a.items; w++) {
+    const TileCtx ctx = Ops::item(a, w);
+    for (int t = 0; t < Ops::NT; t++)
+      Ops::run(a, ctx, t, static_cast<const V*>(in), static_cast<V*>(out));
+  }
+}
+
+template <int B, int NREG>
+static void replay_packed(const Candidate& c, const void* in, void* out) {
+  using Ops = PackedOps<B, NREG>;
+  using E = typename Ops::E;
+  const PackedArgs& a = c.packed;
+  const int nt = c.block;
+  std::vector<E> lds(c.lds_bytes / B + 1);
+  std::vector<typename Ops::Pre> pre(nt);
+  for (int bx = 0; bx < c.grid_x; bx++) {
+    for (int t = 0; t < nt; t++) Ops::prepare(a, bx, t, nt, pre[t]);
+    for (int by = 0; by < c.grid_y; by++) {
+      const int o0 = by * a.outer_per_block;
+      const int o1 = std::min(a.vol_outer, o0 + a.outer_per_block);
+      for (int o = o0; o < o1; o++) {
+        int off_in, off_out;
+        outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
+        for (int t = 0; t < nt; t++)
+          Ops::load(pre[t], off_in, t, nt, static_cast<const E*>(in), lds.data());
+        for (int t = 0; t < nt; t++)
+          Ops::store(pre[t], off_out, t, nt, static_cast<E*>(out), lds.data());
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Instantiation tables
+// ---------------------------------------------------------------------------
+
+struct TiledEntry {
+  int b, tx, ty, vx, vy;
+  const void* kernel;
+  void (*replay)(const Candidate&, const void*, void*);
+};
+struct CopyEntry {
+  int w, tx, np;
+  const void* kernel;
+  void (*replay)(const Candidate&, const void*, void*);
+};
+struct PackedEntry {
+  int b, nreg;
+  const void* kernel;
+  void (*replay)(const Candidate&, const void*, void*);
+};
+
+#define LRT_TILED(B, TX, TY, VX, VY) \
+  {B, TX, TY, VX, VY, reinterpret_cast<const void*>(&k_tiled<B, TX, TY, VX, VY>), &replay_tiled<B, TX, TY, VX, VY>},
+#define LRT_TILED_V(B, TX, TY, V) \
+  LRT_TILED(B, TX, TY, 1, 1) LRT_TILED(B, TX, TY, V, 1) LRT_TILED(B, TX, TY, 1, V) LRT_TILED(B, TX, TY, V, V)
+
+static const TiledEntry kTiledTable[] = {
+    LRT_TILED_V(4, 64, 64, 4) LRT_TILED_V(4, 32, 32, 4) LRT_TILED_V(4, 64, 32, 4) LRT_TILED_V(4, 32, 64, 4)
+    LRT_TILED_V(8, 32, 32, 2) LRT_TILED_V(8, 64, 64, 2) LRT_TILED_V(8, 64, 32, 2) LRT_TILED_V(8, 32, 64, 2)
+    LRT_TILED(16, 16, 16, 1, 1) LRT_TILED(16, 32, 32, 1, 1) LRT_TILED(16, 32, 16, 1, 1) LRT_TILED(16, 16, 32, 1, 1)
+};
+
+#define LRT_COPY(W, TX) \
+  {W, TX, 4, reinterpret_cast<const void*>(&k_copy<W, TX, 4>), &replay_copy<W, TX, 4>},
+#define LRT_COPY_W(W) LRT_COPY(W, 4) LRT_COPY(W, 8) LRT_COPY(W, 16) LRT_COPY(W, 32) LRT_COPY(W, 64)
+
+static const CopyEntry kCopyTable[] = {LRT_COPY_W(4) LRT_COPY_W(8) LRT_COPY_W(16)};
+
+#define LRT_PACKED(B, N) {B, N, reinterpret_cast<const void*>(&k_packed<B, N>), &replay_packed<B, N>},
+#define LRT_PACKED_8(B) \
+  LRT_PACKED(B, 1) LRT_PACKED(B, 2) LRT_PACKED(B, 3) LRT_PACKED(B, 4) LRT_PACKED(B, 5) LRT_PACKED(B, 6) \
+  LRT_PACKED(B, 7) LRT_PACKED(B, 8)
+
+static const PackedEntry kPackedTable[] = {
+    LRT_PACKED_8(4) LRT_PACKED(4, 12) LRT_PACKED(4, 16)
+    LRT_PACKED_8(8) LRT_PACKED(8, 12) LRT_PACKED(8, 16)
+    LRT_PACKED_8(16)
+};
+
+static const TiledEntry* find_tiled(const Candidate& c) {
+  for (const auto& e : kTiledTable)
+    if (e.b == c.elem && e.tx == c.tile_x && e.ty == c.tile_y && e.vx == c.vec_x && e.vy == c.vec_y)
+      return &e;
+  return nullptr;
+}
+static const CopyEntry* find_copy(const Candidate& c) {
+  for (const auto& e : kCopyTable)
+    if (e.w == c.word && e.tx == c.copy_tx) return &e;
+  return nullptr;
+}
+static const PackedEntry* find_packed(const Candidate& c) {
+  for (const auto& e : kPackedTable)
+    if (e.b == c.elem && e.nreg == c.nreg) return &e;
+  return nullptr;
+}
+
+bool tiled_shape_available(int elem, int tx, int ty, int vx, int vy) {
+  for (const auto& e : kTiledTable)
+    if (e.b == elem && e.tx == tx && e.ty == ty && e.vx == vx && e.vy == vy) return true;
+  return false;
+}
+bool copy_shape_available(int word, int tx) {
+  for (const auto& e : kCopyTable)
+    if (e.w == word && e.tx == tx) return true;
+  return false;
+}
+int packed_max_nreg(int elem) {
+  int m = 0;
+  for (const auto& e : kPackedTable)
+    if (e.b == elem && e.nreg > m) m = e.nreg;
+  return m;
+}
+bool packed_nreg_available(int elem, int nreg) {
+  for (const auto& e : kPackedTable)
+    if (e.b == elem && e.nreg == nreg) return true;
+  return false;
+}
+
+static const void* kernel_of(const Candidate& c) {
+  switch (c.method) {
+    case kTiled: {
+      const TiledEntry* e = find_tiled(c);
+      return e ? e->kernel : nullptr;
+    }
+    case kTiledCopy: {
+      const CopyEntry* e = find_copy(c);
+      return e ? e->kernel : nullptr;
+    }
+    case kPacked:
+    case kPackedSplit: {
+      const PackedEntry* e = find_packed(c);
+      return e ? e->kernel : nullptr;
+    }
+    default:
+      return nullptr;
+  }
+}
+
+hipError_t launch_candidate(const Candidate& c, const void* in, void* out, hipStream_t stream) {
+  if (c.method == kTrivial) {
+    return hipMemcpyAsync(out, in, (size_t)(c.bytes / 2), hipMemcpyDeviceToDevice, stream);
+  }
+  const void* k = kernel_of(c);
+  if (!k) return hipErrorInvalidDeviceFunction;
+  dim3 grid(c.grid_x, c.grid_y, 1);
+  dim3 block(c.block, 1, 1);
+  void* args[3];
+  switch (c.method) {
+    case kTiled:
+      args[0] = const_cast<TiledArgs*>(&c.tiled);
+      break;
+    case kTiledCopy:
+      args[0] = const_cast<CopyArgs*>(&c.copy);
+      break;
+    default:
+      args[0] = const_cast<PackedArgs*>(&c.packed);
+      break;
+  }
+  args[1] = &in;
+  args[2] = &out;
+  return hipLaunchKernel(k, grid, block, args, (size_t)((c.method == kPacked || c.method == kPackedSplit) ? c.lds_bytes : 0),
+                         stream);
+}
+
+bool replay_candidate(const Candidate& c, const void* in, void* out) {
+  switch (c.method) {
+    case kTrivial:
+      std::memcpy(out, in, (size_t)(c.bytes / 2));
+      return true;
+    case kTiled: {
+      const TiledEntry* e = find_tiled(c);
+      if (!e) return false;
+      e->replay(c, in, out);
+      return true;
+    }
+    case kTiledCopy: {
+      const CopyEntry* e = find_copy(c);
+      if (!e) return false;
+      e->replay(c, in, out);
+      return true;
+    }
+    case kPacked:
+    case kPackedSplit: {
+      const PackedEntry* e = find_packed(c);
+      if (!e) return false;
+      e->replay(c, in, out);
+      return true;
+    }
+    default:
+      return false;
+  }
+}
+
+int query_occupancy(const Candidate& c) {
+  const void* k = kernel_of(c);
+  if (!k) return 0;
+  if (c.method == kPacked || c.method == kPackedSplit) {
+    // Dynamic LDS above the default per-launch limit needs the attribute.
+    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return 0;
+    }
+  }
+  int n = 0;
+  const size_t lds = (c.method == kPacked || c.method == kPackedSplit) ? c.lds_bytes : 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, c.block, lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int prepare_kernel(const Candidate& c) {
+  if (c.method != kPacked && c.method != kPackedSplit) return 0;
+  const void* k = kernel_of(c);
+  if (!k) return -1;
+  return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes) == hipSuccess ? 0 : -1;
+}
+
+}  // namespace lrt
