@@ -44,6 +44,13 @@ def bufs():
     b.free()
 
 
+@pytest.fixture(scope="module")
+def big_bufs():
+    b = Bufs(600 << 20)
+    yield b
+    b.free()
+
+
 def run_device(dim, perm, elem, x, stream, bufs, index=None, measure=False, offset=(0, 0)):
     nbytes = x.nbytes
     assert nbytes + 256 <= bufs.a.nbytes
@@ -132,7 +139,7 @@ def test_all_perms_rank7_elem8(gpu_stream, bufs):
         assert np.array_equal(out.view(np.uint8), ref.view(np.uint8)), (perm, info["desc"])
 
 
-def test_hand_picked_test3(gpu_stream, bufs):
+def test_hand_picked_test3(gpu_stream, big_bufs):
     # tests/librett_test.cpp:277-417 at full size.
     cases = [
         ([43, 67], [1, 0]),
@@ -147,7 +154,7 @@ def test_hand_picked_test3(gpu_stream, bufs):
     for dim, perm in cases:
         for elem in (4, 8):
             say("case: test3", dim, perm, elem)
-            check(dim, perm, elem, gpu_stream, bufs)
+            check(dim, perm, elem, gpu_stream, big_bufs)
 
 
 def test_plan_measure(gpu_stream, bufs):
@@ -166,12 +173,15 @@ def test_unaligned_pointers_use_fallback(gpu_stream, bufs):
             check(dim, perm, elem, gpu_stream, bufs, offset=off)
 
 
-def test_ten_streams_same_output(bufs):
-    # tests/librett_test.cpp:422-488: ten plans on ten streams into one output.
+def test_ten_streams_same_output():
+    # tests/librett_test.cpp:422-488: ten plans on ten streams into one output
+    # (171M float64 elements, 1.37 GB per tensor).
     dim, perm = [24, 32, 16, 36, 43, 9], [5, 1, 4, 2, 3, 0]
     vol = int(np.prod(dim))
+    say("case: ten streams", dim, perm)
     x = data.uniform(vol, 8, 44)
     ref = oracle.transpose(dim, perm, 8, x, variant="omp")
+    bufs = Bufs(x.nbytes)
     hip.htod(bufs.a.ptr, x)
     streams = [hip.stream_create() for _ in range(10)]
     handles = []
@@ -187,6 +197,7 @@ def test_ten_streams_same_output(bufs):
     for h, s in zip(handles, streams):
         assert L.librettDestroy(h) == L.LIBRETT_SUCCESS
         hip.stream_destroy(s)
+    bufs.free()
 
 
 def test_execute_after_destroy(gpu_stream, bufs):
@@ -210,11 +221,6 @@ NAMED_LARGE = [
 ]
 
 
-@pytest.fixture(scope="module")
-def big_bufs():
-    b = Bufs(272 << 20)
-    yield b
-    b.free()
 
 
 @pytest.mark.parametrize("dim,perm,elem", NAMED_LARGE)
