@@ -65,8 +65,13 @@ typedef struct librett_c_plan_info {
   int grid_x, grid_y, block, lds_bytes, active_blocks;
   double predicted_s;    /* gfx950 model estimate */
   double measured_s;     /* last librett_c_plan_measure time, or -1 */
-  double read_sectors, write_sectors, partial_writes;
+  double read_lines, write_lines, partial_lines; /* modelled 128-byte lines */
   long long bytes;       /* algorithmic bytes, 2 * vol * elem */
+  /* model features: [0] load transactions, [1] store transactions (128-byte
+   * requests of all wave instructions), [2] items (block
+   * rounds), [3] resident blocks, [4] rounds per block, [5] modelled HBM
+   * time, [6] modelled latency time, [7] staged bytes per round */
+  double feat[8];
   char desc[192];
 } librett_c_plan_info;
 
@@ -108,6 +113,23 @@ int librett_c_host_replay(int rank, const int* dim, const int* permutation, size
  * rank, dims and permutation. */
 int librett_c_reduce_ranks(int rank, const int* dim, const int* permutation, int* red_rank, int* red_dim,
                            int* red_perm);
+
+/* Extension: device-to-device copy kernel used as a bandwidth ceiling next
+ * to hipMemcpyAsync (the reference's bench copy kernels, src/GpuMemcpy.cpp,
+ * play the same role). bytes and both pointers must be multiples of 16;
+ * words_per_thread is 1, 2, 4 or 8 16-byte words per round; nontemporal != 0
+ * streams the stores; grid <= 0 launches one block per 256*words_per_thread
+ * words. Asynchronous on `stream`. */
+int librett_c_device_copy(void* dst, const void* src, size_t bytes, void* stream, int words_per_thread,
+                          int nontemporal, int grid);
+
+/* Extension: overrides the analytic model's constants, in this order:
+ * sustained HBM fraction, written-line weight, partial-line weight, round
+ * latency (s), launch overhead (s), seconds per load transaction per CU,
+ * store-transaction weight, per-CU bandwidth (B/s), per-block bandwidth
+ * (B/s). Negative entries keep their value. Affects plans created
+ * afterwards (plans already cached keep their choice). */
+void librett_c_set_model_constants(const double* values, int count);
 
 /* Extension: library version string. */
 const char* librett_c_version(void);

@@ -13,7 +13,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "librett.so")
+# LIBRETT_SO selects another build of the library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("LIBRETT_SO") or os.path.join(_HERE, "librett.so")
 
 LIBRETT_SUCCESS = 0
 LIBRETT_INVALID_PLAN = 1
@@ -38,14 +39,16 @@ class PlanInfo(ctypes.Structure):
         ("grid_x", ctypes.c_int), ("grid_y", ctypes.c_int), ("block", ctypes.c_int), ("lds_bytes", ctypes.c_int),
         ("active_blocks", ctypes.c_int),
         ("predicted_s", ctypes.c_double), ("measured_s", ctypes.c_double),
-        ("read_sectors", ctypes.c_double), ("write_sectors", ctypes.c_double), ("partial_writes", ctypes.c_double),
+        ("read_lines", ctypes.c_double), ("write_lines", ctypes.c_double), ("partial_lines", ctypes.c_double),
         ("bytes", ctypes.c_longlong),
+        ("feat", ctypes.c_double * 8),
         ("desc", ctypes.c_char * 192),
     ]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["desc"] = self.desc.decode()
+        d["feat"] = list(self.feat)
         d["method_name"] = METHOD_NAMES.get(self.method, "?")
         return d
 
@@ -67,7 +70,7 @@ EXPORTED = [
     "librett_c_initialize", "librett_c_finalize", "librett_c_plan", "librett_c_plan_measure",
     "librett_c_destroy", "librett_c_execute", "librett_c_plan_info_get", "librett_c_candidates",
     "librett_c_plan_candidate", "librett_c_host_candidates", "librett_c_host_replay",
-    "librett_c_reduce_ranks", "librett_c_version",
+    "librett_c_reduce_ranks", "librett_c_device_copy", "librett_c_set_model_constants", "librett_c_version",
 ]
 
 
@@ -97,11 +100,14 @@ def lib():
         L.librett_c_host_replay.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, ctypes.POINTER(DeviceDesc),
                                             ctypes.c_int, vp, vp]
         L.librett_c_reduce_ranks.argtypes = [ctypes.c_int, ip, ip, ip, ip, ip]
+        L.librett_c_device_copy.argtypes = [vp, vp, ctypes.c_size_t, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.librett_c_set_model_constants.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.librett_c_version.restype = ctypes.c_char_p
         for name in EXPORTED:
             f = getattr(L, name)
             if name != "librett_c_version" and f.restype is not ctypes.c_char_p:
                 f.restype = ctypes.c_int
+        L.librett_c_set_model_constants.restype = None
         L.librett_c_initialize.restype = None
         L.librett_c_finalize.restype = None
         _LIB = L
@@ -210,6 +216,18 @@ def reduce_ranks(dim, permutation):
     if r != LIBRETT_SUCCESS:
         raise ValueError("reduce_ranks failed: %d" % r)
     return [rd[i] for i in range(rr.value)], [rp[i] for i in range(rr.value)]
+
+
+def device_copy(dst, src, nbytes, stream, words_per_thread=4, nontemporal=0, grid=0):
+    """Hand-written D2D copy kernel (bandwidth ceiling); asynchronous."""
+    return lib().librett_c_device_copy(ctypes.c_void_p(dst), ctypes.c_void_p(src), int(nbytes),
+                                       ctypes.c_void_p(stream), int(words_per_thread), int(nontemporal), int(grid))
+
+
+def set_model_constants(values):
+    """Overrides the analytic model's constants (see librett_c.h)."""
+    arr = (ctypes.c_double * len(values))(*values)
+    lib().librett_c_set_model_constants(arr, len(values))
 
 
 def version():

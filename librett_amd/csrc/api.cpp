@@ -279,10 +279,18 @@ void fill_info(const Candidate& c, double measured, librett_c_plan_info* info) {
   info->active_blocks = c.active_blocks_per_cu;
   info->predicted_s = c.model.predicted_s;
   info->measured_s = measured;
-  info->read_sectors = c.model.read_sectors;
-  info->write_sectors = c.model.write_sectors;
-  info->partial_writes = c.model.partial_writes;
+  info->read_lines = c.model.read_lines;
+  info->write_lines = c.model.write_lines;
+  info->partial_lines = c.model.partial_lines;
   info->bytes = c.bytes;
+  info->feat[0] = c.model.read_tran;
+  info->feat[1] = c.model.write_tran;
+  info->feat[2] = c.model.items;
+  info->feat[3] = c.model.concurrency;
+  info->feat[4] = c.model.steps_per_block;
+  info->feat[5] = c.model.t_hbm;
+  info->feat[6] = c.model.t_latency;
+  info->feat[7] = c.model.items > 0 ? (double)c.bytes / 2 / c.model.items : 0;
   std::snprintf(info->desc, sizeof(info->desc), "%s", c.describe().c_str());
 }
 
@@ -495,6 +503,19 @@ int librett_c_reduce_ranks(int rank, const int* dim, const int* permutation, int
   }
   return LIBRETT_SUCCESS;
 }
+
+int librett_c_device_copy(void* dst, const void* src, size_t bytes, void* stream, int words_per_thread,
+                          int nontemporal, int grid) {
+  const hipError_t e =
+      launch_device_copy(dst, src, bytes, (hipStream_t)stream, words_per_thread, nontemporal, grid);
+  if (e == hipErrorInvalidValue) {
+    (void)hipGetLastError();
+    return LIBRETT_INVALID_PARAMETER;
+  }
+  return e == hipSuccess ? LIBRETT_SUCCESS : LIBRETT_INTERNAL_ERROR;
+}
+
+void librett_c_set_model_constants(const double* values, int count) { set_model_constants(values, count); }
 
 const char* librett_c_version(void) { return "librett-mi355x 0.1 (gfx950)"; }
 

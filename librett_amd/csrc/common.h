@@ -128,6 +128,11 @@ struct CopyArgs {
 struct StagedShape {
   int vol;
   int n;
+  // LDS padding: position p is stored at p + p / pad.d. pad.d is the LDS
+  // stride S of output dim 0 when S is even, so that the write phase, whose
+  // consecutive lanes step by S in LDS, steps by the odd S + 1 instead and
+  // touches distinct banks. ct_sh already includes the padding.
+  FastDiv pad;
   FastDiv din[kMaxMmk];
   int ct_in[kMaxMmk];
   FastDiv dout[kMaxMmk];

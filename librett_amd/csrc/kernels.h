@@ -20,6 +20,11 @@ int query_occupancy(const Candidate& c);
 // Sets per-kernel attributes a launch needs (dynamic LDS size).
 int prepare_kernel(const Candidate& c);
 
+// Device-to-device copy kernel (bandwidth ceiling): k 16-byte words per
+// thread per round, nt != 0 for non-temporal stores, grid <= 0 for one block
+// per 256*k words.
+hipError_t launch_device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream, int k, int nt, int grid);
+
 bool tiled_shape_available(int elem, int tx, int ty, int vx, int vy);
 bool copy_shape_available(int word, int tx);
 int packed_max_nreg(int elem);

@@ -73,6 +73,47 @@ __global__ __launch_bounds__(1024) void k_packed(PackedArgs a, const void* __res
   }
 }
 
+// Plain device copy used as a bandwidth ceiling (the reference's bench copy
+// kernels, src/GpuMemcpy.cpp:42-209, serve the same purpose). Each thread
+// moves K 16-byte words per round, all loads issued before the stores.
+template <int K, bool NT>
+__global__ __launch_bounds__(256) void k_devcopy(const u32x4* __restrict__ in, u32x4* __restrict__ out,
+                                                 long long n16) {
+  const long long stride = (long long)gridDim.x * 256 * K;
+  for (long long b = (long long)blockIdx.x * 256 * K + threadIdx.x; b < n16; b += stride) {
+    u32x4 r[K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (b + k * 256 < n16) r[k] = in[b + k * 256];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (b + k * 256 < n16) {
+        if (NT)
+          __builtin_nontemporal_store(r[k], out + b + k * 256);
+        else
+          out[b + k * 256] = r[k];
+      }
+  }
+}
+
+hipError_t launch_device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream, int k, int nt,
+                              int grid) {
+  if (bytes % 16 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 16 != 0) return hipErrorInvalidValue;
+  const long long n16 = (long long)(bytes / 16);
+  if (n16 == 0) return hipSuccess;
+  const long long per_block = 256LL * k;
+  long long need = (n16 + per_block - 1) / per_block;
+  if (grid <= 0 || grid > need) grid = (int)std::min<long long>(need, 1 << 30);
+  const void* kern = nullptr;
+#define LRT_DC(K_, NT_) \
+  if (k == K_ && nt == NT_) kern = reinterpret_cast<const void*>(&k_devcopy<K_, NT_ != 0>);
+  LRT_DC(1, 0) LRT_DC(2, 0) LRT_DC(4, 0) LRT_DC(8, 0) LRT_DC(1, 1) LRT_DC(2, 1) LRT_DC(4, 1) LRT_DC(8, 1)
+#undef LRT_DC
+  if (!kern) return hipErrorInvalidValue;
+  void* args[3] = {&src, &dst, const_cast<long long*>(&n16)};
+  return hipLaunchKernel(kern, dim3(grid), dim3(256), args, 0, stream);
+}
+
 // ---------------------------------------------------------------------------
 // Host replays: the same per-thread code, barrier order kept by running every
 // thread's load() before any thread's store().

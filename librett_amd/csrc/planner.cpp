@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "kernels.h"
@@ -11,6 +12,17 @@
 namespace lrt {
 
 namespace {
+
+// Blocks per CU that Tiled / TiledCopy grids are capped at (each block loops
+// over the remaining tiles). LIBRETT_GRID_PER_CU overrides it.
+int grid_per_cu() {
+  static int v = [] {
+    const char* e = std::getenv("LIBRETT_GRID_PER_CU");
+    int x = e ? std::atoi(e) : 0;
+    return x > 0 ? x : 64;
+  }();
+  return v;
+}
 
 long long prod(const int* d, int n) {
   long long v = 1;
@@ -61,26 +73,6 @@ void build_outer(const Problem& p, const bool* skip, const long long* cin, const
     m.ct_in[r] = 0;
     m.ct_out[r] = 0;
   }
-}
-
-// 64-byte HBM sectors touched by a contiguous run of `len` elements of `b`
-// bytes starting at element `start` (buffers are at least 256-byte aligned).
-inline long long sectors(long long start, long long len, int b) {
-  if (len <= 0) return 0;
-  long long first = start * b / 64;
-  long long last = ((start + len) * b - 1) / 64;
-  return last - first + 1;
-}
-
-// Sectors of a run that the run covers only in part (first/last sector).
-inline int partial_sectors(long long start, long long len, int b) {
-  if (len <= 0) return 0;
-  long long s0 = start * b, s1 = (start + len) * b;
-  long long first = s0 / 64, last = (s1 - 1) / 64;
-  int n = 0;
-  if (s0 % 64 != 0) n++;
-  if (s1 % 64 != 0 && (last != first || s0 % 64 == 0)) n++;
-  return n;
 }
 
 // Deterministic sample of `k` indices in [0, n).
@@ -302,7 +294,7 @@ void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& 
     a.outer = outer;
     c.block = 256;
     c.lds_bytes = c.tile_y * (c.tile_x + 1) * B;
-    c.grid_x = std::max(1, std::min(a.items, dev.cu_count * 64));
+    c.grid_x = std::max(1, std::min(a.items, dev.cu_count * grid_per_cu()));
     c.bytes = 2 * r.vol * B;
     c.split.sizeMm = 1;
     c.split.sizeMk = 1;
@@ -356,7 +348,7 @@ void add_tiled_copy(const Problem& r, const DeviceInfo& dev, std::vector<Candida
     a.outer = outer;
     c.block = 256;
     c.lds_bytes = 0;
-    c.grid_x = std::max(1, std::min(a.items, dev.cu_count * 64));
+    c.grid_x = std::max(1, std::min(a.items, dev.cu_count * grid_per_cu()));
     c.bytes = 2 * r.vol * B;
     c.split.sizeMm = 1;
     c.split.sizeMk = 2;
@@ -405,13 +397,17 @@ void fill_shape(const Problem& p, const bool* inMmk, const long long* cin, const
     s.n++;
   }
   s.vol = (int)v;
+  // Pad the LDS layout when the stride of output dim 0 is even.
+  const long long S = shs[p.perm[0]];
+  const bool pad = S > 1 && S % 2 == 0;
+  s.pad = make_fastdiv(pad ? (uint32_t)S : 0x7fffffffu);
   int k = 0;
   for (int q = 0; q < p.rank; q++) {
     int d = p.perm[q];
     if (!inMmk[d]) continue;
     s.dout[k] = make_fastdiv((uint32_t)std::max(1, dims[d]));
     s.ct_out[k] = (int)cout[d];
-    s.ct_sh[k] = (int)shs[d];
+    s.ct_sh[k] = (int)((pad && shs[d] >= S) ? shs[d] / S * (S + 1) : shs[d]);
     k++;
   }
   for (int r = s.n; r < kMaxMmk; r++) {
@@ -421,10 +417,17 @@ void fill_shape(const Problem& p, const bool* inMmk, const long long* cin, const
   }
 }
 
+// LDS elements a staged shape occupies with its padding.
+int padded_elems(const StagedShape& s) {
+  if (s.vol <= 0) return 1;
+  return s.vol + (int)fdiv((uint32_t)(s.vol - 1), s.pad) + 1;
+}
+
 bool same_packed(const Candidate& a, const Candidate& b) {
   return a.method == b.method && a.split.volMmk == b.split.volMmk && a.split.volMbar == b.split.volMbar &&
          a.split.volMmkInCont == b.split.volMmkInCont && a.split.volMmkOutCont == b.split.volMmkOutCont &&
-         a.split.numSplit == b.split.numSplit && a.nreg == b.nreg && a.block == b.block;
+         a.split.numSplit == b.split.numSplit && a.split.splitRank == b.split.splitRank &&
+         a.split.volMmkUnsplit == b.split.volMmkUnsplit && a.nreg == b.nreg && a.block == b.block;
 }
 
 void push_packed(const Problem& p, const char* source, const Split& s, int nt, int nreg, const DeviceInfo& dev,
@@ -455,7 +458,7 @@ void push_packed(const Problem& p, const char* source, const Split& s, int nt, i
     a.split_out_stride = 0;
     fill_shape(p, inMmk, cin, cout, -1, 0, a.shape[0]);
     a.shape[1] = a.shape[0];
-    c.lds_bytes = a.shape[0].vol * p.elem;
+    c.lds_bytes = padded_elems(a.shape[0]) * p.elem;
   } else {
     a.num_split = s.numSplit;
     a.split_dim = s.splitDim;
@@ -464,9 +467,9 @@ void push_packed(const Problem& p, const char* source, const Split& s, int nt, i
     const int q = s.splitDim / s.numSplit;
     fill_shape(p, inMmk, cin, cout, s.splitRank, q, a.shape[0]);
     fill_shape(p, inMmk, cin, cout, s.splitRank, q + 1, a.shape[1]);
-    const int maxchunk = (s.splitDim + s.numSplit - 1) / s.numSplit;
-    c.lds_bytes = maxchunk * s.volMmkUnsplit * p.elem;
+    c.lds_bytes = std::max(padded_elems(a.shape[0]), padded_elems(a.shape[1])) * p.elem;
   }
+  if (c.lds_bytes > dev.lds_per_block) return;
   int vol_outer;
   build_outer(p, inMmk, cin, cout, 1, 1, a.outer, vol_outer);
   a.vol_outer = vol_outer;
@@ -483,78 +486,53 @@ void push_packed(const Problem& p, const char* source, const Split& s, int nt, i
 
 void add_packed(const Problem& p, const char* source, const DeviceInfo& dev, std::vector<Candidate>& out) {
   const int B = p.elem;
+  // Distinct staged sets over all (Mm, Mk) splits (many pairs give the same set).
+  std::vector<unsigned> seen;
   for (int numMm = 1; numMm < p.rank; numMm++) {
     for (int numMk = 1; numMk < p.rank; numMk++) {
       Split s;
       split_update(s, numMm, numMk, p);
       if (s.sizeMmk > kMaxMmk) continue;
+      unsigned mask = 0;
+      for (int i = 0; i < numMm; i++) mask |= 1u << i;
+      for (int i = 0; i < numMk; i++) mask |= 1u << p.perm[i];
+      if (std::find(seen.begin(), seen.end(), mask) != seen.end()) continue;
+      seen.push_back(mask);
       const long long bytes = (long long)s.volMmk * B;
       int nt, nreg;
-      if (bytes <= dev.lds_per_block && choose_threads(s.volMmk, B, dev, nt, nreg)) {
+      if (bytes <= dev.lds_per_block && choose_threads(s.volMmk, B, dev, nt, nreg))
         push_packed(p, source, s, nt, nreg, dev, out);
-        continue;
-      }
-      // Split the largest staged dim (first in input order, then output order).
-      Split t = s;
-      int maxDim = 0;
-      for (int i = 0; i < t.sizeMm; i++)
-        if (p.dim[i] > maxDim) {
-          maxDim = p.dim[i];
-          t.splitRank = i;
+      // PackedSplit: chunk one staged dim so that a stage is about 8..128 KiB.
+      // Splitting a dim that starts neither contiguous run keeps both runs
+      // long, so every staged dim is tried, not only the largest (the
+      // reference splits the largest, src/plan.cpp:597-611).
+      for (int d = 0; d < p.rank; d++) {
+        if (!(mask & (1u << d)) || p.dim[d] < 2) continue;
+        Split t = s;
+        t.splitRank = d;
+        t.numSplit = 1;
+        split_update(t, numMm, numMk, p);
+        const long long unsplit_bytes = (long long)t.volMmkUnsplit * B;
+        int last_ns = -1;
+        for (long long target : {8LL << 10, 16LL << 10, 32LL << 10, 64LL << 10, 128LL << 10, 1LL << 40}) {
+          const bool last = target >= dev.lds_per_block;
+          if (last) target = dev.lds_per_block;
+          long long ns = (t.splitDim * unsplit_bytes + target - 1) / target;
+          if (ns < 2) continue;
+          if (ns > t.splitDim) ns = t.splitDim;
+          const int chunk = (int)((t.splitDim + ns - 1) / ns);
+          ns = (t.splitDim + chunk - 1) / chunk;  // fewest splits with this chunk size
+          if (ns < 2 || ns == last_ns) continue;
+          if ((long long)t.splitDim * (ns + 1) >= (1LL << 31)) continue;
+          if ((long long)chunk * unsplit_bytes > dev.lds_per_block) continue;
+          Split u = t;
+          u.numSplit = (int)ns;
+          split_update(u, numMm, numMk, p);
+          if (!choose_threads(chunk * u.volMmkUnsplit, B, dev, nt, nreg)) continue;
+          last_ns = (int)ns;
+          push_packed(p, source, u, nt, nreg, dev, out);
+          if (last) break;
         }
-      for (int i = 0; i < t.sizeMk; i++) {
-        int d = p.perm[i];
-        if (p.dim[d] > maxDim) {
-          maxDim = p.dim[d];
-          t.splitRank = d;
-        }
-      }
-      t.numSplit = 1;
-      split_update(t, numMm, numMk, p);
-      if (t.splitDim < 2) continue;
-      const long long unsplit_bytes = (long long)t.volMmkUnsplit * B;
-      const long long minSplit = (t.splitDim * unsplit_bytes + dev.lds_per_block - 1) / dev.lds_per_block;
-      if (minSplit > t.splitDim || minSplit > 10000) continue;
-      const int lo = (int)std::max<long long>(2, minSplit);
-      const int hi = std::max(lo, std::min(t.splitDim / 2, lo + 60));
-      // Keep: the first split count that launches, the one with the most
-      // staged elements per CU, and the one with the best thread use.
-      int best[3] = {0, 0, 0};
-      long long bestv1 = -1, bestv2 = -1;
-      for (int ns = lo; ns <= hi; ns++) {
-        if ((long long)t.splitDim * (ns + 1) >= (1LL << 31)) break;
-        Split u = t;
-        u.numSplit = ns;
-        split_update(u, numMm, numMk, p);
-        const int chunk = (u.splitDim + ns - 1) / ns;
-        const long long vs = (long long)chunk * u.volMmkUnsplit;
-        if (vs * B > dev.lds_per_block) continue;
-        if (!choose_threads((int)vs, B, dev, nt, nreg)) continue;
-        const int act = std::max(1, std::min(dev.lds_per_cu / (int)(vs * B), dev.max_waves_per_cu / (nt / 64)));
-        const long long used = (long long)(u.splitDim / ns) * u.volMmkUnsplit;
-        const long long v1 = used * act;
-        const long long v2 = (long long)nt * nreg > 0 ? used * 100 / ((long long)nt * nreg) : 0;
-        if (!best[0]) best[0] = ns;
-        if (v1 > bestv1) {
-          bestv1 = v1;
-          best[1] = ns;
-        }
-        if (v2 > bestv2) {
-          bestv2 = v2;
-          best[2] = ns;
-        }
-      }
-      for (int k = 0; k < 3; k++) {
-        if (!best[k]) continue;
-        bool dup = false;
-        for (int j = 0; j < k; j++) dup |= (best[j] == best[k]);
-        if (dup) continue;
-        Split u = t;
-        u.numSplit = best[k];
-        split_update(u, numMm, numMk, p);
-        const int chunk = (u.splitDim + u.numSplit - 1) / u.numSplit;
-        if (!choose_threads(chunk * u.volMmkUnsplit, B, dev, nt, nreg)) continue;
-        push_packed(p, source, u, nt, nreg, dev, out);
       }
     }
   }
@@ -564,20 +542,99 @@ void add_packed(const Problem& p, const char* source, const DeviceInfo& dev, std
 // gfx950 cost model
 // ---------------------------------------------------------------------------
 
+// gfx950 cost model. A candidate's time is the launch overhead plus the
+// largest of four bounds:
+//   HBM      distinct 128-byte lines read and written (counted over a window
+//            of consecutive rounds of one block, see model_packed) at a
+//            sustained fraction of the HBM peak;
+//   CU       the same bytes through the CUs that have work, each limited to
+//            a per-CU bandwidth (small grids cannot fill HBM);
+//   requests 128-byte transactions of all load and store wave instructions
+//            (the reference's gld_tran / gst_tran, src/GpuModel.cpp:50-83)
+//            at a per-CU request rate; store requests weigh more;
+//   rounds   the serial load/store rounds of one resident block, each a
+//            memory latency plus the round's bytes at one block's bandwidth.
+// The constants were fitted (weighted toward each problem's fastest plans)
+// to librettPlanMeasure timings of 3186 candidate plans of 115 problems on
+// MI355X: 55 of about 256 MiB, 60 of 1-32 MiB, at 4, 8 and 16 bytes.
 struct ModelConst {
-  double bw_frac = 0.80;        // sustained fraction of the HBM peak
-  double partial_penalty = 1.0; // extra sector transfers per partially written sector
-  double round_latency = 2.0e-6;// one load+store round of a resident block
-  double launch = 4.0e-6;       // launch + completion overhead
+  double bw_frac = 0.7929;         // sustained fraction of the HBM peak
+  double write_weight = 1.028;     // written line vs read line
+  double partial_weight = 0.0;     // extra weight of a partially written line
+  double round_latency = 1.254e-6; // latency part of one block round (s)
+  double launch = 6.187e-6;        // launch and completion overhead (s)
+  double tran_time = 0.2682e-9;    // seconds per load transaction per CU
+  double store_tran_weight = 33.84;// store transaction vs load transaction
+  double cu_bw = 51.96e9;          // bytes/s one CU can move
+  double block_bw = 18.82e9;       // bytes/s of one block within its rounds
 };
 
-const ModelConst kModel;
+ModelConst g_model;
 
-void model_tiled(Candidate& c, const DeviceInfo& dev) {
+struct Run {
+  long long s, n;  // byte offset and length
+};
+
+// Distinct 128-byte lines a set of runs touches, and how many of them the
+// runs cover only in part (a run boundary falls inside the line).
+void line_stats(std::vector<Run>& runs, double& lines, double& partial) {
+  lines = partial = 0;
+  if (runs.empty()) return;
+  std::sort(runs.begin(), runs.end(), [](const Run& a, const Run& b) { return a.s < b.s; });
+  long long last_line = -1, last_partial = -1;
+  size_t i = 0;
+  while (i < runs.size()) {
+    long long s0 = runs[i].s, e0 = runs[i].s + runs[i].n;
+    size_t j = i + 1;
+    while (j < runs.size() && runs[j].s <= e0) {
+      e0 = std::max(e0, runs[j].s + runs[j].n);
+      j++;
+    }
+    const long long first = s0 >> 7, last = (e0 - 1) >> 7;
+    lines += (double)(last - first + 1) - (first == last_line ? 1 : 0);
+    if ((s0 & 127) && first != last_partial) {
+      partial += 1;
+      last_partial = first;
+    }
+    if ((e0 & 127) && last != last_partial) {
+      partial += 1;
+      last_partial = last;
+    }
+    last_line = last;
+    i = j;
+  }
+}
+
+// 128-byte transactions of one run of `n` bytes at byte offset `s` (one
+// wave instruction touching it issues one request per line it spans).
+inline double tran(long long s, long long n) { return n <= 0 ? 0 : (double)(((s + n - 1) >> 7) - (s >> 7) + 1); }
+
+// Transactions of a staged shape's runs (element offsets) for one round:
+// runs are walked in LDS order (input side) or output order, cut at 64-lane
+// wave boundaries.
+double staged_tran(const std::vector<std::pair<long long, int>>& runs, long long base, int B) {
+  double t = 0;
+  long long pos = 0;  // position in walk order
+  for (const auto& r : runs) {
+    long long s = r.first, n = r.second;
+    while (n > 0) {
+      const long long room = 64 - (pos & 63);
+      const long long k = std::min<long long>(room, n);
+      t += tran((base + s) * B, k * B);
+      s += k;
+      n -= k;
+      pos += k;
+    }
+  }
+  return t;
+}
+
+void model_tiled(Candidate& c) {
   const TiledArgs& a = c.tiled;
   const int B = c.elem;
-  double R = 0, W = 0, P = 0;
-  std::vector<long long> ws = sample_indices(a.items, 48);
+  double R = 0, W = 0, P = 0, TR = 0, TW = 0;
+  std::vector<long long> ws = sample_indices(a.items, 24);
+  std::vector<Run> rin, rout;
   for (long long w : ws) {
     uint32_t mb = (uint32_t)(w / a.ntiles);
     int tile = (int)(w % a.ntiles);
@@ -585,27 +642,36 @@ void model_tiled(Candidate& c, const DeviceInfo& dev) {
     int oi, oo;
     outer_offsets(a.outer, mb, oi, oo);
     int tx = std::min(c.tile_x, a.X - x0), ty = std::min(c.tile_y, a.Y - y0);
-    for (int y = 0; y < ty; y++) R += sectors(oi + x0 + (long long)(y0 + y) * a.in_ys, tx, B);
-    for (int x = 0; x < tx; x++) {
-      long long s = oo + y0 + (long long)(x0 + x) * a.out_xs;
-      W += sectors(s, ty, B);
-      P += partial_sectors(s, ty, B);
-    }
+    rin.clear();
+    rout.clear();
+    for (int y = 0; y < ty; y++) rin.push_back({(oi + x0 + (long long)(y0 + y) * a.in_ys) * B, (long long)tx * B});
+    for (int x = 0; x < tx; x++) rout.push_back({(oo + y0 + (long long)(x0 + x) * a.out_xs) * B, (long long)ty * B});
+    for (const Run& r : rin) TR += tran(r.s, r.n);
+    for (const Run& r : rout) TW += tran(r.s, r.n);
+    double l, p;
+    line_stats(rin, l, p);
+    R += l;
+    line_stats(rout, l, p);
+    W += l;
+    P += p;
   }
   const double scale = ws.empty() ? 0 : (double)a.items / ws.size();
-  c.model.read_sectors = R * scale;
-  c.model.write_sectors = W * scale;
-  c.model.partial_writes = P * scale;
+  c.model.read_lines = R * scale;
+  c.model.write_lines = W * scale;
+  c.model.partial_lines = P * scale;
+  c.model.read_tran = TR * scale;
+  c.model.write_tran = TW * scale;
   c.model.items = a.items;
   c.model.steps_per_block = (double)a.items / c.grid_x;
 }
 
-void model_copy(Candidate& c, const DeviceInfo& dev) {
+void model_copy(Candidate& c) {
   const CopyArgs& a = c.copy;
   const int Wb = c.word;
   const int ty_tile = (256 / c.copy_tx) * 4;
-  double R = 0, W = 0, P = 0;
-  std::vector<long long> ws = sample_indices(a.items, 48);
+  double R = 0, W = 0, P = 0, TR = 0, TW = 0;
+  std::vector<long long> ws = sample_indices(a.items, 24);
+  std::vector<Run> rin, rout;
   for (long long w : ws) {
     uint32_t mb = (uint32_t)(w / a.ntiles);
     int tile = (int)(w % a.ntiles);
@@ -613,23 +679,33 @@ void model_copy(Candidate& c, const DeviceInfo& dev) {
     int oi, oo;
     outer_offsets(a.outer, mb, oi, oo);
     int tx = std::min(c.copy_tx, a.X - x0), ty = std::min(ty_tile, a.Y - y0);
+    rin.clear();
+    rout.clear();
     for (int y = 0; y < ty; y++) {
-      R += sectors(oi + x0 + (long long)(y0 + y) * a.in_ys, tx, Wb);
-      long long s = oo + x0 + (long long)(y0 + y) * a.out_ys;
-      W += sectors(s, tx, Wb);
-      P += partial_sectors(s, tx, Wb);
+      rin.push_back({(oi + x0 + (long long)(y0 + y) * a.in_ys) * Wb, (long long)tx * Wb});
+      rout.push_back({(oo + x0 + (long long)(y0 + y) * a.out_ys) * Wb, (long long)tx * Wb});
     }
+    for (const Run& r : rin) TR += tran(r.s, r.n);
+    for (const Run& r : rout) TW += tran(r.s, r.n);
+    double l, p;
+    line_stats(rin, l, p);
+    R += l;
+    line_stats(rout, l, p);
+    W += l;
+    P += p;
   }
   const double scale = ws.empty() ? 0 : (double)a.items / ws.size();
-  c.model.read_sectors = R * scale;
-  c.model.write_sectors = W * scale;
-  c.model.partial_writes = P * scale;
+  c.model.read_lines = R * scale;
+  c.model.write_lines = W * scale;
+  c.model.partial_lines = P * scale;
+  c.model.read_tran = TR * scale;
+  c.model.write_tran = TW * scale;
   c.model.items = a.items;
   c.model.steps_per_block = (double)a.items / c.grid_x;
 }
 
-// Contiguous runs (start offset, length) of one staged shape walked in LDS
-// order (input side) or output order (output side).
+// Contiguous runs (start offset, length in elements) of one staged shape
+// walked in LDS order (input side) or output order (output side).
 void staged_runs(const StagedShape& s, bool output_side, std::vector<std::pair<long long, int>>& runs) {
   runs.clear();
   const int n = s.n;
@@ -660,80 +736,111 @@ void staged_runs(const StagedShape& s, bool output_side, std::vector<std::pair<l
   if (run_len > 0) runs.push_back({run_start, run_len});
 }
 
-void model_packed(Candidate& c, const DeviceInfo& dev) {
+// Packed / PackedSplit: lines are counted over a window of consecutive outer
+// indices of one block, because a block walks its outer range in order and
+// the L2 keeps the lines its previous rounds touched; short runs that are
+// adjacent across rounds therefore cost one line fetch, not one per run.
+void model_packed(Candidate& c) {
   const PackedArgs& a = c.packed;
   const int B = c.elem;
-  double R = 0, W = 0, P = 0;
+  double R = 0, W = 0, P = 0, TR = 0, TW = 0, rounds = 0;
   std::vector<std::pair<long long, int>> rin[2], rout[2];
   const int nvar = (c.method == kPackedSplit) ? 2 : 1;
   for (int v = 0; v < nvar; v++) {
     staged_runs(a.shape[v], false, rin[v]);
     staged_runs(a.shape[v], true, rout[v]);
   }
-  // Sample (split chunk, outer index) pairs.
-  const long long nitems = (long long)a.num_split * a.vol_outer;
-  std::vector<long long> ws = sample_indices(nitems, 8);
-  for (long long w : ws) {
-    int split = (int)(w % a.num_split);
-    uint32_t o = (uint32_t)(w / a.num_split);
+  const long long nblocks = (long long)c.grid_x * c.grid_y;
+  const int window = std::max(1, std::min(4, a.outer_per_block));
+  std::vector<long long> bs = sample_indices(nblocks, 6);
+  std::vector<Run> win, wout;
+  for (long long b : bs) {
+    const int split = (int)(b % c.grid_x);
+    const int by = (int)(b / c.grid_x);
     const int q0 = a.split_dim / a.num_split;
     const int c0 = (int)((long long)split * a.split_dim / a.num_split);
     const int c1 = (int)((long long)(split + 1) * a.split_dim / a.num_split);
     const int plus = (nvar == 2) ? (c1 - c0) - q0 : 0;
-    int oi, oo;
-    outer_offsets(a.outer, o, oi, oo);
-    const long long bi = (long long)c0 * a.split_in_stride + oi;
-    const long long bo = (long long)c0 * a.split_out_stride + oo;
-    for (const auto& r : rin[plus]) R += sectors(bi + r.first, r.second, B);
-    for (const auto& r : rout[plus]) {
-      W += sectors(bo + r.first, r.second, B);
-      P += partial_sectors(bo + r.first, r.second, B);
+    const int o0 = by * a.outer_per_block;
+    const int o1 = std::min(a.vol_outer, o0 + window);
+    win.clear();
+    wout.clear();
+    for (int o = o0; o < o1; o++) {
+      int oi, oo;
+      outer_offsets(a.outer, (uint32_t)o, oi, oo);
+      const long long bi = (long long)c0 * a.split_in_stride + oi;
+      const long long bo = (long long)c0 * a.split_out_stride + oo;
+      for (const auto& r : rin[plus]) win.push_back({(bi + r.first) * B, (long long)r.second * B});
+      for (const auto& r : rout[plus]) wout.push_back({(bo + r.first) * B, (long long)r.second * B});
+      TR += staged_tran(rin[plus], bi, B);
+      TW += staged_tran(rout[plus], bo, B);
     }
+    double l, p;
+    line_stats(win, l, p);
+    R += l;
+    line_stats(wout, l, p);
+    W += l;
+    P += p;
+    rounds += o1 - o0;
   }
-  const double scale = ws.empty() ? 0 : (double)nitems / ws.size();
-  c.model.read_sectors = R * scale;
-  c.model.write_sectors = W * scale;
-  c.model.partial_writes = P * scale;
+  const long long nitems = (long long)a.num_split * a.vol_outer;
+  const double scale = rounds > 0 ? (double)nitems / rounds : 0;
+  c.model.read_lines = R * scale;
+  c.model.write_lines = W * scale;
+  c.model.partial_lines = P * scale;
+  c.model.read_tran = TR * scale;
+  c.model.write_tran = TW * scale;
   c.model.items = (double)nitems;
   c.model.steps_per_block = a.outer_per_block;
 }
 
 }  // namespace
 
+void set_model_constants(const double* v, int n) {
+  double* f[] = {&g_model.bw_frac,   &g_model.write_weight,      &g_model.partial_weight,
+                 &g_model.round_latency, &g_model.launch,        &g_model.tran_time,
+                 &g_model.store_tran_weight, &g_model.cu_bw,     &g_model.block_bw};
+  for (int i = 0; i < n && i < 9; i++)
+    if (v[i] >= 0) *f[i] = v[i];
+}
+
 void model_candidate(Candidate& c, const Problem& full, const DeviceInfo& dev) {
   ModelStats& m = c.model;
   m = ModelStats();
+  const ModelConst& K = g_model;
   if (c.method == kTrivial) {
-    m.read_sectors = m.write_sectors = (double)(c.bytes / 2) / 64.0;
-    m.t_hbm = (double)c.bytes / (dev.hbm_bytes_per_s * kModel.bw_frac);
-    m.t_launch = kModel.launch;
+    m.read_lines = m.write_lines = (double)(c.bytes / 2) / 128.0;
+    m.t_hbm = (double)c.bytes / (dev.hbm_bytes_per_s * K.bw_frac);
+    m.t_launch = K.launch;
+    m.items = 1;
     m.predicted_s = m.t_hbm + m.t_launch;
     return;
   }
   if (c.active_blocks_per_cu <= 0) c.active_blocks_per_cu = active_blocks(c, dev);
   switch (c.method) {
     case kTiled:
-      model_tiled(c, dev);
+      model_tiled(c);
       break;
     case kTiledCopy:
-      model_copy(c, dev);
+      model_copy(c);
       break;
     default:
-      model_packed(c, dev);
+      model_packed(c);
       break;
   }
   const double blocks = (double)c.grid_x * c.grid_y;
   m.concurrency = std::min(blocks, (double)dev.cu_count * std::max(1, c.active_blocks_per_cu));
-  const double sector_bytes = 64.0 * (m.read_sectors + m.write_sectors + kModel.partial_penalty * m.partial_writes);
-  // Bandwidth reachable with the resident blocks: each resident block keeps
-  // about one round of its staged bytes in flight.
-  const double bytes_per_round = (double)c.bytes / std::max(1.0, m.items);
-  const double inflight = m.concurrency * bytes_per_round;
-  const double bw_cap = inflight / kModel.round_latency;
-  const double bw = std::min(dev.hbm_bytes_per_s * kModel.bw_frac, std::max(bw_cap, 1e9));
-  m.t_hbm = sector_bytes / bw;
-  m.t_latency = std::ceil(m.items / std::max(1.0, m.concurrency)) * kModel.round_latency * 0.5;
-  m.t_launch = kModel.launch;
+  const double line_bytes =
+      128.0 * (m.read_lines + K.write_weight * m.write_lines + K.partial_weight * m.partial_lines);
+  const double cus = std::min((double)dev.cu_count, m.concurrency);
+  const double t_hbm = line_bytes / (dev.hbm_bytes_per_s * K.bw_frac);
+  const double t_cu = line_bytes / (std::max(1.0, cus) * K.cu_bw);
+  const double t_req = (m.read_tran + K.store_tran_weight * m.write_tran) * K.tran_time / dev.cu_count;
+  const double round_bytes = (double)c.bytes / std::max(1.0, m.items);
+  const double rounds = std::ceil(m.items / std::max(1.0, m.concurrency));
+  m.t_hbm = std::max(std::max(t_hbm, t_cu), t_req);
+  m.t_latency = rounds * (K.round_latency + round_bytes / K.block_bw);
+  m.t_launch = K.launch;
   m.predicted_s = std::max(m.t_hbm, m.t_latency) + m.t_launch;
   (void)full;
 }

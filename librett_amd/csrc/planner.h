@@ -64,10 +64,12 @@ void split_update(Split& s, int sizeMm, int sizeMk, const Problem& p);
 
 // Modelled cost and the features it was computed from.
 struct ModelStats {
-  double read_sectors = 0;    // 64-byte HBM sectors read
-  double write_sectors = 0;   // 64-byte HBM sectors written
-  double partial_writes = 0;  // written sectors not fully covered by one block
-  double items = 0;           // blocks x inner steps
+  double read_lines = 0;      // distinct 128-byte lines read (per block window)
+  double write_lines = 0;     // distinct 128-byte lines written
+  double partial_lines = 0;   // written lines a block window covers only in part
+  double read_tran = 0;       // 128-byte transactions of all load wave instructions
+  double write_tran = 0;      // 128-byte transactions of all store wave instructions
+  double items = 0;           // block rounds (tiles or staged sub-tensors)
   double concurrency = 0;     // blocks resident on the device at once
   double steps_per_block = 0; // serial load/store rounds per resident block
   double t_hbm = 0, t_latency = 0, t_launch = 0;
@@ -107,6 +109,10 @@ std::vector<Candidate> make_candidates(const Problem& full, const DeviceInfo& de
 
 // Applies the gfx950 cost model to a candidate (fills cand.model).
 void model_candidate(Candidate& c, const Problem& full, const DeviceInfo& dev);
+
+// Overrides the model constants (bw_frac, write_weight, partial_weight,
+// inflight_half, round_latency, launch); entries <= 0 keep their value.
+void set_model_constants(const double* v, int n);
 
 // Index of the candidate with the smallest modelled time (Trivial wins, as
 // in operator> of src/plan.cpp:721-768).

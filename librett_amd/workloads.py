@@ -1,0 +1,86 @@
+"""Seeded random transpose problems for benchmarks, tuning and training.
+
+Shapes follow the published workload descriptions: a rank, dims whose
+product lands near a target volume, and a random non-identity permutation.
+Classes for the large single transposes:
+  keep  : perm[0] == 0 and dim0 in [64, 4096]   (TiledCopy territory)
+  move  : perm[0] != 0, dim0 and dim[perm[0]] >= 64 (Tiled territory)
+  small : dim0 in [2, 8], perm[0] != 0          (Packed / PackedSplit)
+"""
+import math
+
+import numpy as np
+
+
+def _perm(rng, rank, first=None, avoid_identity=True):
+    while True:
+        if first is None:
+            p = [int(x) for x in rng.permutation(rank)]
+        else:
+            rest = [int(x) for x in rng.permutation([i for i in range(rank) if i != first])]
+            p = [first] + rest
+        if not avoid_identity or p != list(range(rank)) or rank == 1:
+            return p
+
+
+def _fill_dims(rng, rank, fixed, target):
+    """Dims (log-uniform) for positions not in `fixed` so the volume is near target."""
+    free = [i for i in range(rank) if i not in fixed]
+    dims = [0] * rank
+    rest = target
+    for i, v in fixed.items():
+        dims[i] = v
+        rest /= v
+    rest = max(rest, 2.0 ** len(free))
+    if free:
+        w = rng.uniform(0.3, 1.0, size=len(free))
+        w = w / w.sum()
+        for i, wi in zip(free, w):
+            dims[i] = max(2, int(round(rest ** wi)))
+        # correct the largest free dim toward the target volume
+        vol = math.prod(dims)
+        j = max(free, key=lambda k: dims[k])
+        dims[j] = max(2, int(round(dims[j] * target / vol)))
+    return dims
+
+
+def large_case(seed, elem, cls, target_bytes=256 << 20):
+    """One ~256 MiB single-transpose problem of class keep/move/small."""
+    rng = np.random.default_rng(seed)
+    target = target_bytes // elem
+    rank = int(rng.integers(2, 7))
+    if cls == "keep":
+        rank = max(rank, 3)
+        d0 = int(2 ** rng.uniform(6, 12))
+        dims = _fill_dims(rng, rank, {0: d0}, target)
+        perm = _perm(rng, rank, first=0)
+    elif cls == "move":
+        while True:
+            perm = _perm(rng, rank)
+            if perm[0] != 0:
+                break
+        d0 = int(2 ** rng.uniform(6, 12))
+        if rank == 2:
+            dims = [d0, max(64, int(round(target / d0)))]
+        else:
+            dp = int(2 ** rng.uniform(6, 12))
+            dims = _fill_dims(rng, rank, {0: d0, perm[0]: dp}, target)
+    else:
+        rank = max(rank, 3)
+        while True:
+            perm = _perm(rng, rank)
+            if perm[0] != 0:
+                break
+        dims = _fill_dims(rng, rank, {0: int(rng.integers(2, 9))}, target)
+    return dims, perm
+
+
+def random_problem(rng, rank_lo, rank_hi, vol_lo, vol_hi, elems):
+    """Rank U{lo..hi}, volume log-uniform in [vol_lo, vol_hi] elements,
+    element size drawn from `elems`, random non-identity permutation."""
+    rank = int(rng.integers(rank_lo, rank_hi + 1))
+    elem = int(rng.choice(elems))
+    target = math.exp(rng.uniform(math.log(vol_lo), math.log(vol_hi)))
+    dims = _fill_dims(rng, rank, {}, target)
+    perm = _perm(rng, rank)
+    return dims, perm, elem
