@@ -133,28 +133,42 @@ def main():
     dom_name, (dom_ms, dom_bytes, dom_launches) = max(by_kernel.items(), key=lambda kv: kv[1][0])
     dom_gbs = dom_bytes / (dom_ms * 1e-3) / 1e9
 
-    # Same-run D2D ceiling: hipMemcpyAsync of the largest case's input.
-    big = max(cases, key=lambda c: c["bytes"])
-    nb = big["bytes"] // 2
-    e0, e1 = hip.Event(), hip.Event()
-    for _ in range(3):
-        hip.memcpy_d2d_async(big["b"].ptr, big["a"].ptr, nb, stream)
-    d2d = []
-    for _ in range(10):
-        e0.record(stream)
-        hip.memcpy_d2d_async(big["b"].ptr, big["a"].ptr, nb, stream)
-        e1.record(stream)
-        e1.synchronize()
-        d2d.append(e1.elapsed_ms(e0))
-    d2d.sort()
-    d2d_gbs = 2 * nb / (d2d[len(d2d) // 2] * 1e-3) / 1e9
+    # Ceilings measured the same way: each transpose of a step replaced by a
+    # copy of the same bytes between the same buffers, for the same steps.
+    def ceiling(copy_fn):
+        for c in cases:
+            copy_fn(c)
+        hip.stream_sync(stream)
+        per = [[] for _ in cases]
+        ev = [[(hip.Event(), hip.Event()) for _ in range(args.steps)] for _ in cases]
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            for i, c in enumerate(cases):
+                ev[i][k][0].record(stream)
+                copy_fn(c)
+                ev[i][k][1].record(stream)
+        hip.stream_sync(stream)
+        wall_c = time.perf_counter() - t0
+        for i in range(len(cases)):
+            per[i] = sum(ev[i][k][1].elapsed_ms(ev[i][k][0]) for k in range(args.steps)) / args.steps
+        return step_bytes * args.steps / wall_c / 1e9, per
+
+    d2d_gbs, d2d_per = ceiling(lambda c: hip.memcpy_d2d_async(c["b"].ptr, c["a"].ptr, c["bytes"] // 2, stream))
+    devcopy_gbs, dc_per = ceiling(
+        lambda c: L.device_copy(c["b"].ptr, c["a"].ptr, c["bytes"] // 2 // 16 * 16, stream, 1, 1, 0))
+    for i, c in enumerate(per_case):
+        c["d2d_gbs"] = round(cases[i]["bytes"] / (d2d_per[i] * 1e-3) / 1e9, 1)
+        c["copy_kernel_gbs"] = round(cases[i]["bytes"] / (dc_per[i] * 1e-3) / 1e9, 1)
 
     for c in per_case:
-        c["frac_d2d"] = round(c["gbs"] / d2d_gbs, 3)
-        log("case %-28s %-20s %2d B %-11s %8.1f GB/s (%5.1f%% of D2D, %5.1f%% of 8 TB/s) model %.3f ms, run %.3f ms"
-            % (tuple(c["dim"]), tuple(c["perm"]), c["elem"], c["method"], c["gbs"], 100 * c["frac_d2d"],
-               100 * c["gbs"] / HBM_PEAK_GBS, c["predicted_ms"], c["avg_ms"]))
-    log("D2D hipMemcpyAsync %.1f GB/s; aggregate %.1f GB/s (%.1f%% of D2D)" % (d2d_gbs, value, 100 * value / d2d_gbs))
+        c["frac_d2d"] = round(c["gbs"] / c["d2d_gbs"], 3)
+        c["frac_copy_kernel"] = round(c["gbs"] / c["copy_kernel_gbs"], 3)
+        log("case %-28s %-20s %2d B %-11s %8.1f GB/s (%5.1f%% of D2D, %5.1f%% of copy kernel, %5.1f%% of 8 TB/s) "
+            "model %.3f ms, run %.3f ms" % (tuple(c["dim"]), tuple(c["perm"]), c["elem"], c["method"], c["gbs"],
+                                            100 * c["frac_d2d"], 100 * c["frac_copy_kernel"],
+                                            100 * c["gbs"] / HBM_PEAK_GBS, c["predicted_ms"], c["avg_ms"]))
+    log("D2D hipMemcpyAsync %.1f GB/s, streaming copy kernel %.1f GB/s; aggregate %.1f GB/s (%.1f%% of D2D, "
+        "%.1f%% of the copy kernel)" % (d2d_gbs, devcopy_gbs, value, 100 * value / d2d_gbs, 100 * value / devcopy_gbs))
 
     cpu = None
     if not args.no_cpu:
@@ -187,6 +201,8 @@ def main():
         },
         "d2d_copy_gbs": round(d2d_gbs, 1),
         "frac_of_d2d": round(value / d2d_gbs, 3),
+        "copy_kernel_gbs": round(devcopy_gbs, 1),
+        "frac_of_copy_kernel": round(value / devcopy_gbs, 3),
         "roofline": {
             "bound": "hbm",
             "kernel": dom_name,

@@ -108,6 +108,8 @@ bool device_info(DeviceInfo& out) {
   d.max_threads_per_block = prop.maxThreadsPerBlock;
   d.warp_size = prop.warpSize;
   d.max_waves_per_cu = std::max(1, prop.maxThreadsPerMultiProcessor / std::max(1, prop.warpSize));
+  // l2CacheSize is the L2 of one XCD; MI355X has one XCD per 32 CUs.
+  if (prop.l2CacheSize > 0) d.l2_bytes = (double)prop.l2CacheSize * std::max(1, d.cu_count / 32);
   d.has_gpu = 1;
   g_devices[dev] = d;
   out = d;

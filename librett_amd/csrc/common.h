@@ -100,6 +100,15 @@ struct TiledArgs {
   FastDiv ntx_div;
   FastDiv ntiles_div;
   int items;         // ntiles * volMbar
+  // Tile order: tiles go in groups of `group` tile rows (along y); within a
+  // group, y varies fastest, so blocks dispatched together write adjacent
+  // stretches of the same output rows as well as reading adjacent stretches
+  // of the same input rows.
+  int group;
+  FastDiv group_div;      // group
+  FastDiv group_row_div;  // group * ntx
+  FastDiv last_div;       // rows in the last (possibly partial) group
+  int last_group_start;   // first tile index of the last group
   OuterMap outer;
 };
 

@@ -96,8 +96,17 @@ struct TiledOps {
     TileCtx c;
     uint32_t mb = fdiv(w, a.ntiles_div);
     uint32_t tile = w - mb * (uint32_t)a.ntiles;
-    uint32_t ty = fdiv(tile, a.ntx_div);
-    uint32_t tx = tile - ty * (uint32_t)a.ntx;
+    uint32_t tx, ty;
+    if (a.group <= 1) {
+      ty = fdiv(tile, a.ntx_div);
+      tx = tile - ty * (uint32_t)a.ntx;
+    } else {
+      const uint32_t g = fdiv(tile, a.group_row_div);  // group index
+      const uint32_t within = tile - g * a.group_row_div.d;
+      const FastDiv& rows = (tile >= (uint32_t)a.last_group_start) ? a.last_div : a.group_div;
+      tx = fdiv(within, rows);
+      ty = g * (uint32_t)a.group + (within - tx * rows.d);
+    }
     c.x0 = (int)tx * TX;
     c.y0 = (int)ty * TY;
     outer_offsets(a.outer, mb, c.off_in, c.off_out);

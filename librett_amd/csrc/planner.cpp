@@ -24,6 +24,18 @@ int grid_per_cu() {
   return v;
 }
 
+// Tile-row group of the Tiled order (16 measured best for 256 MiB f32 and
+// f64 2-D transposes on MI355X: +20-25% over row-major tile order in a
+// steady stream of transposes); LIBRETT_TILE_GROUP overrides it.
+int tile_group() {
+  static int v = [] {
+    const char* e = std::getenv("LIBRETT_TILE_GROUP");
+    int x = e ? std::atoi(e) : 0;
+    return x > 0 ? x : 16;
+  }();
+  return v;
+}
+
 long long prod(const int* d, int n) {
   long long v = 1;
   for (int i = 0; i < n; i++) v *= d[i];
@@ -93,6 +105,17 @@ std::vector<long long> sample_indices(long long n, int k) {
 }
 
 }  // namespace
+
+void set_tile_group(TiledArgs& a, int nty, int group) {
+  group = std::max(1, std::min(group, nty));
+  a.group = group;
+  a.group_div = make_fastdiv((uint32_t)group);
+  a.group_row_div = make_fastdiv((uint32_t)(group * a.ntx));
+  const int ngroups = (nty + group - 1) / group;
+  const int last_rows = nty - (ngroups - 1) * group;
+  a.last_div = make_fastdiv((uint32_t)last_rows);
+  a.last_group_start = (ngroups - 1) * group * a.ntx;
+}
 
 Problem canonicalize(int rank, const int* dim, const int* perm, int elem) {
   Problem p;
@@ -291,6 +314,7 @@ void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& 
     a.ntx_div = make_fastdiv((uint32_t)a.ntx);
     a.ntiles_div = make_fastdiv((uint32_t)a.ntiles);
     a.items = (int)std::min<long long>((long long)a.ntiles * vol_outer, 0x7fffffffLL);
+    set_tile_group(a, nty, tile_group());
     a.outer = outer;
     c.block = 256;
     c.lds_bytes = c.tile_y * (c.tile_x + 1) * B;
@@ -555,18 +579,20 @@ void add_packed(const Problem& p, const char* source, const DeviceInfo& dev, std
 //   rounds   the serial load/store rounds of one resident block, each a
 //            memory latency plus the round's bytes at one block's bandwidth.
 // The constants were fitted (weighted toward each problem's fastest plans)
-// to librettPlanMeasure timings of 3186 candidate plans of 115 problems on
-// MI355X: 55 of about 256 MiB, 60 of 1-32 MiB, at 4, 8 and 16 bytes.
+// to librettPlanMeasure-style timings of about 3200 candidate plans of 115
+// problems on MI355X (55 of about 256 MiB, 60 of 1-64 MiB, 4/8/16-byte
+// elements), each timed in a steady stream over buffers that do not fit the
+// 256 MiB MALL, so every run pays for its own HBM reads and write-backs.
 struct ModelConst {
-  double bw_frac = 0.7929;         // sustained fraction of the HBM peak
-  double write_weight = 1.028;     // written line vs read line
+  double bw_frac = 0.8392;         // sustained fraction of the HBM peak
+  double write_weight = 1.4047;    // written line vs read line
   double partial_weight = 0.0;     // extra weight of a partially written line
-  double round_latency = 1.254e-6; // latency part of one block round (s)
-  double launch = 6.187e-6;        // launch and completion overhead (s)
-  double tran_time = 0.2682e-9;    // seconds per load transaction per CU
-  double store_tran_weight = 33.84;// store transaction vs load transaction
-  double cu_bw = 51.96e9;          // bytes/s one CU can move
-  double block_bw = 18.82e9;       // bytes/s of one block within its rounds
+  double round_latency = 1.325e-6; // latency part of one block round (s)
+  double launch = 3.998e-6;        // launch and completion overhead (s)
+  double tran_time = 0.1974e-9;    // seconds per load transaction per CU
+  double store_tran_weight = 63.65;// store transaction vs load transaction
+  double cu_bw = 27.39e9;          // bytes/s one CU can move
+  double block_bw = 15.06e9;       // bytes/s of one block within its rounds
 };
 
 ModelConst g_model;
@@ -740,10 +766,10 @@ void staged_runs(const StagedShape& s, bool output_side, std::vector<std::pair<l
 // indices of one block, because a block walks its outer range in order and
 // the L2 keeps the lines its previous rounds touched; short runs that are
 // adjacent across rounds therefore cost one line fetch, not one per run.
-void model_packed(Candidate& c) {
+void model_packed(Candidate& c, const DeviceInfo& dev) {
   const PackedArgs& a = c.packed;
   const int B = c.elem;
-  double R = 0, W = 0, P = 0, TR = 0, TW = 0, rounds = 0;
+  double R = 0, W = 0, P = 0, TR = 0, TW = 0, rounds = 0, R1 = 0, W1 = 0;
   std::vector<std::pair<long long, int>> rin[2], rout[2];
   const int nvar = (c.method == kPackedSplit) ? 2 : 1;
   for (int v = 0; v < nvar; v++) {
@@ -782,11 +808,31 @@ void model_packed(Candidate& c) {
     W += l;
     P += p;
     rounds += o1 - o0;
+    // Same rounds without reuse between rounds (each round alone).
+    for (int o = o0; o < o1; o++) {
+      int oi, oo;
+      outer_offsets(a.outer, (uint32_t)o, oi, oo);
+      const long long bi = (long long)c0 * a.split_in_stride + oi;
+      const long long bo = (long long)c0 * a.split_out_stride + oo;
+      win.clear();
+      wout.clear();
+      for (const auto& r : rin[plus]) win.push_back({(bi + r.first) * B, (long long)r.second * B});
+      for (const auto& r : rout[plus]) wout.push_back({(bo + r.first) * B, (long long)r.second * B});
+      line_stats(win, l, p);
+      R1 += l;
+      line_stats(wout, l, p);
+      W1 += l;
+    }
   }
   const long long nitems = (long long)a.num_split * a.vol_outer;
   const double scale = rounds > 0 ? (double)nitems / rounds : 0;
-  c.model.read_lines = R * scale;
-  c.model.write_lines = W * scale;
+  // Reuse between rounds needs the lines of one round of all resident blocks
+  // to stay in L2; past that footprint it fades out.
+  const double resident = std::min((double)c.grid_x * c.grid_y, (double)dev.cu_count * std::max(1, c.active_blocks_per_cu));
+  const double footprint = resident * 128.0 * (R1 + W1) / std::max(1.0, rounds);
+  const double keep = std::min(1.0, dev.l2_bytes / std::max(1.0, footprint));
+  c.model.read_lines = (R * keep + R1 * (1 - keep)) * scale;
+  c.model.write_lines = (W * keep + W1 * (1 - keep)) * scale;
   c.model.partial_lines = P * scale;
   c.model.read_tran = TR * scale;
   c.model.write_tran = TW * scale;
@@ -825,7 +871,7 @@ void model_candidate(Candidate& c, const Problem& full, const DeviceInfo& dev) {
       model_copy(c);
       break;
     default:
-      model_packed(c);
+      model_packed(c, dev);
       break;
   }
   const double blocks = (double)c.grid_x * c.grid_y;

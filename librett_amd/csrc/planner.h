@@ -26,6 +26,7 @@ struct DeviceInfo {
   int warp_size = 64;
   int max_waves_per_cu = 32;
   double hbm_bytes_per_s = 8.0e12;   // published MI355X HBM3E peak
+  double l2_bytes = 32.0 * 1024 * 1024;  // L2 of all XCDs (8 x 4 MiB on MI355X)
   int has_gpu = 0;                   // occupancy may be queried from the runtime
 };
 
@@ -117,6 +118,10 @@ void set_model_constants(const double* v, int n);
 // Index of the candidate with the smallest modelled time (Trivial wins, as
 // in operator> of src/plan.cpp:721-768).
 int choose_by_model(const std::vector<Candidate>& cands);
+
+// Sets the Tiled tile order: groups of `group` tile rows, y fastest inside a
+// group (group 1 = x fastest).
+void set_tile_group(TiledArgs& a, int nty, int group);
 
 // Variant of a Tiled candidate without vector accesses, used when the
 // caller's pointers are not aligned to the vector width.
