@@ -51,6 +51,10 @@ def main():
     ap.add_argument("--measure", action="store_true", help="choose plans with librettPlanMeasure")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--workload", default="large", choices=["large", "batched"],
+                    help="large: the seven ~256 MiB cases (default); batched: 4096 uneven problems")
+    ap.add_argument("--problems", type=int, default=4096, help="batched workload: number of problems")
+    ap.add_argument("--tune", type=int, default=3, help="batched workload: autotuner candidates per problem")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X and has no "
@@ -62,6 +66,8 @@ def main():
 
     hip.check(hip.runtime().hipSetDevice(0), "hipSetDevice")
     stream = hip.stream_create()
+    if args.workload == "batched":
+        return bench_batched(args, stream)
 
     # One random source buffer, uploaded as the input of every case.
     max_bytes = max(int(np.prod(d)) * e for d, _, e in CASES)
@@ -217,6 +223,119 @@ def main():
         "cpu_baseline": cpu,
         "cases": per_case,
     }
+    print(json.dumps(line), flush=True)
+
+
+def batched_problems(n, seed=20261202):
+    """Uneven batch: rank 3-5, volume log-uniform 1e3..1e6 elements, float64
+    or complex128, random non-identity permutation per problem."""
+    from librett_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    return [W.random_problem(rng, 3, 5, 1e3, 1e6, [8, 16]) for _ in range(n)]
+
+
+def bench_batched(args, stream):
+    """Batched workload: the whole batch in one persistent graph launch vs a
+    loop of per-problem librettExecute calls on the same buffers."""
+    import librett_amd as L
+    from librett_amd import data, hip
+    probs = batched_problems(args.problems)
+    sizes = [int(np.prod(d)) * e for d, p, e in probs]
+    offs, o = [], 0
+    for s_ in sizes:
+        offs.append(o)
+        o += (s_ + 255) // 256 * 256
+    total = o
+    alg = sum(2 * s_ for s_ in sizes)
+    log("batched: %d problems, %.2f GB algorithmic, %.2f GB per side" % (len(probs), alg / 1e9, total / 1e9))
+    A = hip.DeviceBuffer(total)
+    B = hip.DeviceBuffer(total)
+    chunk = 256 << 20
+    src = data.uniform(chunk // 8, 8, 7).view(np.uint8)
+    hip.htod(A.ptr, src[:min(chunk, total)])
+    done = min(chunk, total)
+    while done < total:
+        n = min(done, total - done)
+        hip.memcpy_d2d_async(A.ptr + done, A.ptr, n, stream)
+        done += n
+    hip.memset(B.ptr, 0, total)
+    hip.stream_sync(stream)
+    ins = [A.ptr + o for o in offs]
+    outs = [B.ptr + o for o in offs]
+    e0, e1 = hip.Event(), hip.Event()
+
+    def timed(fn, steps, warm):
+        for _ in range(warm):
+            fn()
+        hip.stream_sync(stream)
+        e0.record(stream)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return (time.perf_counter() - t0) / steps, e1.elapsed_ms(e0) * 1e-3 / steps
+
+    # Per-call loop of librettExecute with plans built beforehand.
+    t0 = time.perf_counter()
+    handles = []
+    for d, p, e in probs:
+        r, h = L.librettPlan(len(d), list(d), list(p), e, stream)
+        if r != L.LIBRETT_SUCCESS:
+            raise SystemExit("plan failed")
+        handles.append(h)
+    plan_loop_s = time.perf_counter() - t0
+
+    def loop():
+        for h, i, o_ in zip(handles, ins, outs):
+            L.librettExecute(h, i, o_)
+    wall_loop, ev_loop = timed(loop, max(1, args.steps // 4), 1)
+    log("per-call loop: %.2f ms/step (%.1f GB/s), planning %.1f s" % (wall_loop * 1e3, alg / wall_loop / 1e9,
+                                                                       plan_loop_s))
+    res = {}
+    for tune in (0, args.tune):
+        t0 = time.perf_counter()
+        bt = L.Batch(probs, stream, tune=tune)
+        bt.set_buffers(ins, outs)
+        info = bt.info()
+        build_s = time.perf_counter() - t0
+        wall_b, ev_b = timed(lambda: bt.execute(use_graph=True), args.steps, args.warmup)
+        wall_n, ev_n = timed(lambda: bt.execute(use_graph=False), args.steps, args.warmup)
+        res[tune] = dict(wall=wall_b, ev=ev_b, wall_plain=wall_n, info=info, build_s=build_s)
+        log("batch tune=%d: graph %.3f ms (%.1f GB/s), plain launch %.3f ms; units %d grid %d; methods %s; "
+            "plan %.2f s tune %.2f s" % (tune, wall_b * 1e3, alg / wall_b / 1e9, wall_n * 1e3, info["total_units"],
+                                        info["grid"], info["method_count"], info["plan_seconds"],
+                                        info["tune_seconds"]))
+        bt.destroy()
+    best_tune = min(res, key=lambda k: res[k]["wall"])
+    wall = res[best_tune]["wall"]
+    value = alg / wall / 1e9
+    for h in handles:
+        L.librettDestroy(h)
+    line = {
+        "metric": "batched transpose GB/s (2*bytes/t), one persistent graph launch per batch",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+c128 (opaque 8/16-byte moves)",
+        "data": "synthetic",
+        "config": {"workload": "batched: %d uneven transposes, rank 3-5, vol 1e3-1e6, f64/c128" % len(probs),
+                   "bytes_per_step": alg, "autotune_candidates": best_tune},
+        "per_call_loop_gbs": round(alg / wall_loop / 1e9, 1),
+        "speedup_vs_per_call": round(wall_loop / wall, 2),
+        "model_only_gbs": round(alg / res[0]["wall"] / 1e9, 1),
+        "autotuned_gbs": round(alg / res[args.tune]["wall"] / 1e9, 1),
+        "plain_launch_gbs": round(alg / res[best_tune]["wall_plain"] / 1e9, 1),
+        "batch_info": res[best_tune]["info"],
+    }
+    A.free()
+    B.free()
     print(json.dumps(line), flush=True)
 
 

@@ -131,6 +131,45 @@ int librett_c_device_copy(void* dst, const void* src, size_t bytes, void* stream
  * afterwards (plans already cached keep their choice). */
 void librett_c_set_model_constants(const double* values, int count);
 
+/* ---------------------------------------------------------------------
+ * Batched engine (extension; the reference has no batched mode). Many
+ * independent transposes run in one persistent kernel launch, replayed from
+ * a HIP graph. Problems are given flattened: problem i has ranks[i] dims at
+ * dims[o_i ...] and its permutation at perms[o_i ...], o_i = sum_{j<i}
+ * ranks[j]. tune = 0 picks each problem's plan with the cost model; tune = k
+ * > 0 times the model's k best batch plans of each distinct problem inside
+ * the persistent kernel and keeps the fastest (remembered across batches).
+ * ------------------------------------------------------------------- */
+typedef struct librett_c_batch_info {
+  int count;            /* problems */
+  int descriptors;      /* device descriptors (a flat copy may use two) */
+  int total_units;      /* work units of one launch */
+  int grid;             /* persistent workgroups */
+  int tuned;            /* problems timed by the autotuner in this batch */
+  int method_count[6];  /* problems per method (index = method number) */
+  long long bytes;      /* algorithmic bytes, sum of 2 * vol * elem */
+  double plan_seconds, tune_seconds;
+} librett_c_batch_info;
+
+int librett_c_batch_plan(unsigned int* handle, int count, const int* ranks, const int* dims, const int* perms,
+                         const size_t* elems, void* stream, int tune);
+
+/* Device buffers of every problem (idata[i], odata[i]); may be called again
+ * to point the batch at other buffers. Synchronises the batch's stream. */
+int librett_c_batch_set_buffers(unsigned int handle, const void* const* idata, void* const* odata);
+
+/* Enqueues the whole batch on the stream given at planning: one graph
+ * launch (use_graph != 0, captured on first use) or one plain launch. */
+int librett_c_batch_execute(unsigned int handle, int use_graph);
+
+int librett_c_batch_destroy(unsigned int handle);
+int librett_c_batch_info_get(unsigned int handle, librett_c_batch_info* info);
+
+/* Host replay of the batch's persistent kernel for a described device (no
+ * GPU): host buffers, `grid` workgroups (<= 0: the planned grid). */
+int librett_c_batch_host_replay(int count, const int* ranks, const int* dims, const int* perms, const size_t* elems,
+                                const librett_c_device* dev, const void* const* idata, void* const* odata, int grid);
+
 /* Extension: library version string. */
 const char* librett_c_version(void);
 

@@ -71,7 +71,22 @@ EXPORTED = [
     "librett_c_destroy", "librett_c_execute", "librett_c_plan_info_get", "librett_c_candidates",
     "librett_c_plan_candidate", "librett_c_host_candidates", "librett_c_host_replay",
     "librett_c_reduce_ranks", "librett_c_device_copy", "librett_c_set_model_constants", "librett_c_version",
+    "librett_c_batch_plan", "librett_c_batch_set_buffers", "librett_c_batch_execute", "librett_c_batch_destroy",
+    "librett_c_batch_info_get", "librett_c_batch_host_replay",
 ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_int), ("descriptors", ctypes.c_int), ("total_units", ctypes.c_int), ("grid", ctypes.c_int),
+        ("tuned", ctypes.c_int), ("method_count", ctypes.c_int * 6), ("bytes", ctypes.c_longlong),
+        ("plan_seconds", ctypes.c_double), ("tune_seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["method_count"] = {METHOD_NAMES.get(i, str(i)): self.method_count[i] for i in range(6) if self.method_count[i]}
+        return d
 
 
 def lib():
@@ -102,6 +117,15 @@ def lib():
         L.librett_c_reduce_ranks.argtypes = [ctypes.c_int, ip, ip, ip, ip, ip]
         L.librett_c_device_copy.argtypes = [vp, vp, ctypes.c_size_t, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.librett_c_set_model_constants.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        L.librett_c_batch_plan.argtypes = [up, ctypes.c_int, ip, ip, ip, szp, vp, ctypes.c_int]
+        L.librett_c_batch_set_buffers.argtypes = [ctypes.c_uint, pp, pp]
+        L.librett_c_batch_execute.argtypes = [ctypes.c_uint, ctypes.c_int]
+        L.librett_c_batch_destroy.argtypes = [ctypes.c_uint]
+        L.librett_c_batch_info_get.argtypes = [ctypes.c_uint, ctypes.POINTER(BatchInfo)]
+        L.librett_c_batch_host_replay.argtypes = [ctypes.c_int, ip, ip, ip, szp, ctypes.POINTER(DeviceDesc), pp, pp,
+                                                  ctypes.c_int]
         L.librett_c_version.restype = ctypes.c_char_p
         for name in EXPORTED:
             f = getattr(L, name)
@@ -228,6 +252,68 @@ def set_model_constants(values):
     """Overrides the analytic model's constants (see librett_c.h)."""
     arr = (ctypes.c_double * len(values))(*values)
     lib().librett_c_set_model_constants(arr, len(values))
+
+
+def _flatten(problems):
+    ranks = [len(d) for d, _, _ in problems]
+    dims = [x for d, _, _ in problems for x in d]
+    perms = [x for _, p, _ in problems for x in p]
+    elems = [e for _, _, e in problems]
+    return ((ctypes.c_int * len(ranks))(*ranks), _iarr(dims), _iarr(perms), (ctypes.c_size_t * len(elems))(*elems))
+
+
+def _ptrs(v):
+    return (ctypes.c_void_p * len(v))(*[ctypes.c_void_p(int(x)) for x in v])
+
+
+class Batch:
+    """Many transposes in one persistent launch (extension of the API).
+
+    ``problems`` is a list of (dims, perm, elem). ``tune`` = 0 uses the cost
+    model, k > 0 times the model's k best batch plans of each problem."""
+
+    def __init__(self, problems, stream, tune=0):
+        self.problems = list(problems)
+        h = ctypes.c_uint(0)
+        r = lib().librett_c_batch_plan(ctypes.byref(h), len(self.problems), *_flatten(self.problems),
+                                       ctypes.c_void_p(stream), int(tune))
+        if r != LIBRETT_SUCCESS:
+            raise ValueError("batch plan failed: %d" % r)
+        self.handle = h.value
+
+    def set_buffers(self, inputs, outputs):
+        r = lib().librett_c_batch_set_buffers(self.handle, _ptrs(inputs), _ptrs(outputs))
+        if r != LIBRETT_SUCCESS:
+            raise ValueError("batch set_buffers failed: %d" % r)
+
+    def execute(self, use_graph=True):
+        return lib().librett_c_batch_execute(self.handle, 1 if use_graph else 0)
+
+    def info(self):
+        info = BatchInfo()
+        r = lib().librett_c_batch_info_get(self.handle, ctypes.byref(info))
+        if r != LIBRETT_SUCCESS:
+            raise ValueError("batch info failed: %d" % r)
+        return info.as_dict()
+
+    def destroy(self):
+        if self.handle:
+            lib().librett_c_batch_destroy(self.handle)
+            self.handle = 0
+
+
+def batch_host_replay(problems, inputs, grid=0, device=None):
+    """Host replay of the batched kernel; `inputs` are numpy arrays."""
+    import numpy as np
+    dev = device or mi355x_desc()
+    ins = [np.ascontiguousarray(x) for x in inputs]
+    outs = [np.zeros_like(x) for x in ins]
+    r = lib().librett_c_batch_host_replay(len(problems), *_flatten(problems), ctypes.byref(dev),
+                                          _ptrs([x.ctypes.data for x in ins]), _ptrs([y.ctypes.data for y in outs]),
+                                          int(grid))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("batch host replay failed: %d" % r)
+    return outs
 
 
 def version():

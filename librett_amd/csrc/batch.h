@@ -1,0 +1,51 @@
+// Batched engine: many independent transposes in one persistent launch.
+//
+// Every problem gets a plan from the batch-mode planner (256-thread
+// workgroups, small LDS, a fixed set of kernel shapes) and a descriptor in
+// one device arena. The work of all problems is a single index space of
+// units (tiles, row tiles, or a few staged rounds), laid out problem after
+// problem; workgroup g of the persistent kernel takes units g, g + G,
+// g + 2G, ... so neighbouring units of one problem run at the same time on
+// different CUs, and no counters or atomics are needed (the launch can be
+// replayed from a HIP graph as is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "planner.h"
+
+namespace lrt {
+
+struct BatchDesc {
+  int variant;       // index into the persistent kernel's shape table
+  int units;         // work units of this problem
+  int rounds;        // Packed: outer indices per unit
+  int pad_;
+  FastDiv split_div; // Packed: number of chunks of the split dim
+  const void* in;
+  void* out;
+  union Args {
+    TiledArgs tiled;
+    CopyArgs copy;
+    PackedArgs packed;
+  } a;
+};
+
+// Shape-table index of a batch-mode candidate, or -1.
+int batch_variant(const Candidate& c);
+
+// Fills d (except pointers) for a candidate; returns false if unsupported.
+bool batch_describe(const Candidate& c, BatchDesc& d);
+
+// Launches the persistent kernel over `total` units of `nprob` problems.
+// `start` holds the first unit of every problem plus `total` at the end.
+hipError_t launch_batch(const BatchDesc* d_descs, const int* d_start, int nprob, int total, int grid,
+                        hipStream_t stream);
+
+// Resident workgroups per CU of the persistent kernel (runtime query).
+int batch_occupancy();
+
+// Host replay of the persistent kernel (descriptors hold host pointers).
+void replay_batch(const BatchDesc* descs, const int* start, int nprob, int total, int grid);
+
+}  // namespace lrt

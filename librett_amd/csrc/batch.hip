@@ -1,0 +1,315 @@
+// Persistent kernel of the batched engine, its shape table and host replay.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "batch.h"
+#include "kernel_ops.h"
+#include "kernels.h"
+
+namespace lrt {
+
+namespace {
+
+// Shape table: method, element bytes, then per method
+//   Tiled:     tile_x, tile_y, vec_x, vec_y
+//   TiledCopy: word, copy_tx
+//   Packed:    nreg
+#define LRT_BATCH_TABLE(X)                                              \
+  X(0, kTiled, 4, 64, 64, 1, 1) X(1, kTiled, 4, 64, 64, 4, 1)           \
+  X(2, kTiled, 4, 64, 64, 1, 4) X(3, kTiled, 4, 64, 64, 4, 4)           \
+  X(4, kTiled, 8, 32, 32, 1, 1) X(5, kTiled, 8, 32, 32, 2, 1)           \
+  X(6, kTiled, 8, 32, 32, 1, 2) X(7, kTiled, 8, 32, 32, 2, 2)           \
+  X(8, kTiled, 16, 32, 32, 1, 1)                                        \
+  X(9, kTiledCopy, 4, 4, 16, 0, 0) X(10, kTiledCopy, 4, 4, 64, 0, 0)    \
+  X(11, kTiledCopy, 8, 8, 16, 0, 0) X(12, kTiledCopy, 8, 8, 64, 0, 0)   \
+  X(13, kTiledCopy, 16, 16, 16, 0, 0) X(14, kTiledCopy, 16, 16, 64, 0, 0) \
+  X(15, kPacked, 4, 1, 0, 0, 0) X(16, kPacked, 4, 2, 0, 0, 0)           \
+  X(17, kPacked, 4, 4, 0, 0, 0) X(18, kPacked, 4, 8, 0, 0, 0)           \
+  X(19, kPacked, 8, 1, 0, 0, 0) X(20, kPacked, 8, 2, 0, 0, 0)           \
+  X(21, kPacked, 8, 4, 0, 0, 0) X(22, kPacked, 8, 8, 0, 0, 0)           \
+  X(23, kPacked, 16, 1, 0, 0, 0) X(24, kPacked, 16, 2, 0, 0, 0)         \
+  X(25, kPacked, 16, 4, 0, 0, 0)
+
+struct BatchShape {
+  int id, method, b, p1, p2, p3, p4;
+};
+#define LRT_BATCH_ROW(ID, M, B, P1, P2, P3, P4) {ID, M, B, P1, P2, P3, P4},
+const BatchShape kBatchShapes[] = {LRT_BATCH_TABLE(LRT_BATCH_ROW)};
+#undef LRT_BATCH_ROW
+
+__device__ __forceinline__ int batch_find(const int* __restrict__ start, int nprob, int u) {
+  int lo = 0, hi = nprob;  // start[lo] <= u < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= u)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int B, int TX, int TY, int VX, int VY>
+__device__ __forceinline__ void dev_tiled(const BatchDesc& D, int lu, unsigned char* lds_raw) {
+  using Ops = TiledOps<B, TX, TY, VX, VY>;
+  using E = typename Ops::E;
+  E* lds = reinterpret_cast<E*>(lds_raw);
+  const TileCtx c = Ops::item(D.a.tiled, (uint32_t)lu);
+  Ops::load(D.a.tiled, c, threadIdx.x, static_cast<const E*>(D.in), lds);
+  __syncthreads();
+  Ops::store(D.a.tiled, c, threadIdx.x, static_cast<E*>(D.out), lds);
+  __syncthreads();
+}
+
+template <int W, int TX>
+__device__ __forceinline__ void dev_copy(const BatchDesc& D, int lu) {
+  using Ops = CopyOps<W, TX, 4>;
+  using V = typename Ops::V;
+  const TileCtx c = Ops::item(D.a.copy, (uint32_t)lu);
+  Ops::run(D.a.copy, c, threadIdx.x, static_cast<const V*>(D.in), static_cast<V*>(D.out));
+}
+
+template <int B, int NREG>
+__device__ __forceinline__ void dev_packed(const BatchDesc& D, int lu, unsigned char* lds_raw) {
+  using Ops = PackedOps<B, NREG>;
+  using E = typename Ops::E;
+  E* lds = reinterpret_cast<E*>(lds_raw);
+  const PackedArgs& a = D.a.packed;
+  const uint32_t og = fdiv((uint32_t)lu, D.split_div);
+  const int split = lu - (int)(og * D.split_div.d);
+  typename Ops::Pre p;
+  Ops::prepare(a, split, threadIdx.x, 256, p);
+  const int o0 = (int)og * D.rounds;
+  const int o1 = min(a.vol_outer, o0 + D.rounds);
+  for (int o = o0; o < o1; o++) {
+    int off_in, off_out;
+    outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
+    Ops::load(p, off_in, threadIdx.x, 256, static_cast<const E*>(D.in), lds);
+    __syncthreads();
+    Ops::store(p, off_out, threadIdx.x, 256, static_cast<E*>(D.out), lds);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch(const BatchDesc* __restrict__ descs, const int* __restrict__ start,
+                                               int nprob, int total) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[kBatchLds];
+  int u = blockIdx.x;
+  if (u >= total) return;
+  int p = batch_find(start, nprob, u);
+  for (; u < total; u += gridDim.x) {
+    while (start[p + 1] <= u) p++;
+    const BatchDesc& D = descs[p];
+    const int lu = u - start[p];
+    switch (D.variant) {
+      case 0: dev_tiled<4, 64, 64, 1, 1>(D, lu, lds); break;
+      case 1: dev_tiled<4, 64, 64, 4, 1>(D, lu, lds); break;
+      case 2: dev_tiled<4, 64, 64, 1, 4>(D, lu, lds); break;
+      case 3: dev_tiled<4, 64, 64, 4, 4>(D, lu, lds); break;
+      case 4: dev_tiled<8, 32, 32, 1, 1>(D, lu, lds); break;
+      case 5: dev_tiled<8, 32, 32, 2, 1>(D, lu, lds); break;
+      case 6: dev_tiled<8, 32, 32, 1, 2>(D, lu, lds); break;
+      case 7: dev_tiled<8, 32, 32, 2, 2>(D, lu, lds); break;
+      case 8: dev_tiled<16, 32, 32, 1, 1>(D, lu, lds); break;
+      case 9: dev_copy<4, 16>(D, lu); break;
+      case 10: dev_copy<4, 64>(D, lu); break;
+      case 11: dev_copy<8, 16>(D, lu); break;
+      case 12: dev_copy<8, 64>(D, lu); break;
+      case 13: dev_copy<16, 16>(D, lu); break;
+      case 14: dev_copy<16, 64>(D, lu); break;
+      case 15: dev_packed<4, 1>(D, lu, lds); break;
+      case 16: dev_packed<4, 2>(D, lu, lds); break;
+      case 17: dev_packed<4, 4>(D, lu, lds); break;
+      case 18: dev_packed<4, 8>(D, lu, lds); break;
+      case 19: dev_packed<8, 1>(D, lu, lds); break;
+      case 20: dev_packed<8, 2>(D, lu, lds); break;
+      case 21: dev_packed<8, 4>(D, lu, lds); break;
+      case 22: dev_packed<8, 8>(D, lu, lds); break;
+      case 23: dev_packed<16, 1>(D, lu, lds); break;
+      case 24: dev_packed<16, 2>(D, lu, lds); break;
+      case 25: dev_packed<16, 4>(D, lu, lds); break;
+      default: break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host replay: the same per-thread code, workgroup by workgroup.
+// ---------------------------------------------------------------------------
+
+template <int B, int TX, int TY, int VX, int VY>
+void host_tiled(const BatchDesc& D, int lu, unsigned char* lds_raw) {
+  using Ops = TiledOps<B, TX, TY, VX, VY>;
+  using E = typename Ops::E;
+  E* lds = reinterpret_cast<E*>(lds_raw);
+  const TileCtx c = Ops::item(D.a.tiled, (uint32_t)lu);
+  for (int t = 0; t < 256; t++) Ops::load(D.a.tiled, c, t, static_cast<const E*>(D.in), lds);
+  for (int t = 0; t < 256; t++) Ops::store(D.a.tiled, c, t, static_cast<E*>(D.out), lds);
+}
+
+template <int W, int TX>
+void host_copy(const BatchDesc& D, int lu) {
+  using Ops = CopyOps<W, TX, 4>;
+  using V = typename Ops::V;
+  const TileCtx c = Ops::item(D.a.copy, (uint32_t)lu);
+  for (int t = 0; t < 256; t++) Ops::run(D.a.copy, c, t, static_cast<const V*>(D.in), static_cast<V*>(D.out));
+}
+
+template <int B, int NREG>
+void host_packed(const BatchDesc& D, int lu, unsigned char* lds_raw) {
+  using Ops = PackedOps<B, NREG>;
+  using E = typename Ops::E;
+  E* lds = reinterpret_cast<E*>(lds_raw);
+  const PackedArgs& a = D.a.packed;
+  const uint32_t og = fdiv((uint32_t)lu, D.split_div);
+  const int split = lu - (int)(og * D.split_div.d);
+  std::vector<typename Ops::Pre> pre(256);
+  for (int t = 0; t < 256; t++) Ops::prepare(a, split, t, 256, pre[t]);
+  const int o0 = (int)og * D.rounds;
+  const int o1 = std::min(a.vol_outer, o0 + D.rounds);
+  for (int o = o0; o < o1; o++) {
+    int off_in, off_out;
+    outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
+    for (int t = 0; t < 256; t++) Ops::load(pre[t], off_in, t, 256, static_cast<const E*>(D.in), lds);
+    for (int t = 0; t < 256; t++) Ops::store(pre[t], off_out, t, 256, static_cast<E*>(D.out), lds);
+  }
+}
+
+void host_unit(const BatchDesc& D, int lu, unsigned char* lds) {
+  switch (D.variant) {
+    case 0: host_tiled<4, 64, 64, 1, 1>(D, lu, lds); break;
+    case 1: host_tiled<4, 64, 64, 4, 1>(D, lu, lds); break;
+    case 2: host_tiled<4, 64, 64, 1, 4>(D, lu, lds); break;
+    case 3: host_tiled<4, 64, 64, 4, 4>(D, lu, lds); break;
+    case 4: host_tiled<8, 32, 32, 1, 1>(D, lu, lds); break;
+    case 5: host_tiled<8, 32, 32, 2, 1>(D, lu, lds); break;
+    case 6: host_tiled<8, 32, 32, 1, 2>(D, lu, lds); break;
+    case 7: host_tiled<8, 32, 32, 2, 2>(D, lu, lds); break;
+    case 8: host_tiled<16, 32, 32, 1, 1>(D, lu, lds); break;
+    case 9: host_copy<4, 16>(D, lu); break;
+    case 10: host_copy<4, 64>(D, lu); break;
+    case 11: host_copy<8, 16>(D, lu); break;
+    case 12: host_copy<8, 64>(D, lu); break;
+    case 13: host_copy<16, 16>(D, lu); break;
+    case 14: host_copy<16, 64>(D, lu); break;
+    case 15: host_packed<4, 1>(D, lu, lds); break;
+    case 16: host_packed<4, 2>(D, lu, lds); break;
+    case 17: host_packed<4, 4>(D, lu, lds); break;
+    case 18: host_packed<4, 8>(D, lu, lds); break;
+    case 19: host_packed<8, 1>(D, lu, lds); break;
+    case 20: host_packed<8, 2>(D, lu, lds); break;
+    case 21: host_packed<8, 4>(D, lu, lds); break;
+    case 22: host_packed<8, 8>(D, lu, lds); break;
+    case 23: host_packed<16, 1>(D, lu, lds); break;
+    case 24: host_packed<16, 2>(D, lu, lds); break;
+    case 25: host_packed<16, 4>(D, lu, lds); break;
+    default: break;
+  }
+}
+
+}  // namespace
+
+bool batch_tiled_available(int elem, int tx, int ty, int vx, int vy) {
+  for (const auto& s : kBatchShapes)
+    if (s.method == kTiled && s.b == elem && s.p1 == tx && s.p2 == ty && s.p3 == vx && s.p4 == vy) return true;
+  return false;
+}
+
+bool batch_copy_available(int word, int tx) {
+  for (const auto& s : kBatchShapes)
+    if (s.method == kTiledCopy && s.p1 == word && s.p2 == tx) return true;
+  return false;
+}
+
+bool batch_packed_available(int elem, int nreg) {
+  for (const auto& s : kBatchShapes)
+    if (s.method == kPacked && s.b == elem && s.p1 == nreg) return true;
+  return false;
+}
+
+int batch_variant(const Candidate& c) {
+  for (const auto& s : kBatchShapes) {
+    switch (c.method) {
+      case kTiled:
+        if (s.method == kTiled && s.b == c.elem && s.p1 == c.tile_x && s.p2 == c.tile_y && s.p3 == c.vec_x &&
+            s.p4 == c.vec_y)
+          return s.id;
+        break;
+      case kTiledCopy:
+        if (s.method == kTiledCopy && s.p1 == c.word && s.p2 == c.copy_tx) return s.id;
+        break;
+      case kPacked:
+      case kPackedSplit:
+        if (s.method == kPacked && s.b == c.elem && s.p1 == c.nreg && c.block == 256) return s.id;
+        break;
+      default:
+        break;
+    }
+  }
+  return -1;
+}
+
+bool batch_describe(const Candidate& c, BatchDesc& d) {
+  d = BatchDesc();
+  d.variant = batch_variant(c);
+  if (d.variant < 0) return false;
+  d.split_div = make_fastdiv(1);
+  d.rounds = 1;
+  switch (c.method) {
+    case kTiled:
+      d.a.tiled = c.tiled;
+      d.units = c.tiled.items;
+      break;
+    case kTiledCopy:
+      d.a.copy = c.copy;
+      d.units = c.copy.items;
+      break;
+    default: {
+      d.a.packed = c.packed;
+      // Units of about 48 KiB of staged traffic.
+      const long long round_bytes = 2LL * std::max(1, c.packed.shape[0].vol) * c.elem;
+      const int rounds = (int)std::max<long long>(1, std::min<long long>(c.packed.vol_outer, (48 << 10) / round_bytes));
+      d.rounds = rounds;
+      const int groups = (c.packed.vol_outer + rounds - 1) / rounds;
+      d.split_div = make_fastdiv((uint32_t)c.packed.num_split);
+      d.units = groups * c.packed.num_split;
+      break;
+    }
+  }
+  return d.units > 0;
+}
+
+hipError_t launch_batch(const BatchDesc* d_descs, const int* d_start, int nprob, int total, int grid,
+                        hipStream_t stream) {
+  if (total <= 0) return hipSuccess;
+  void* args[4] = {&d_descs, &d_start, &nprob, &total};
+  return hipLaunchKernel(reinterpret_cast<const void*>(&k_batch), dim3(grid), dim3(256), args, 0, stream);
+}
+
+int batch_occupancy() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&k_batch), 256, 0) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+void replay_batch(const BatchDesc* descs, const int* start, int nprob, int total, int grid) {
+  std::vector<unsigned char> lds(kBatchLds + 64);
+  for (int g = 0; g < grid; g++) {
+    int u = g;
+    if (u >= total) continue;
+    int p = 0;
+    while (start[p + 1] <= u) p++;
+    for (; u < total; u += grid) {
+      while (start[p + 1] <= u) p++;
+      host_unit(descs[p], u - start[p], lds.data());
+    }
+  }
+}
+
+}  // namespace lrt

@@ -26,6 +26,10 @@ int prepare_kernel(const Candidate& c);
 hipError_t launch_device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream, int k, int nt, int grid);
 
 bool tiled_shape_available(int elem, int tx, int ty, int vx, int vy);
+// Shapes the batched engine's persistent kernel is instantiated for (batch.hip).
+bool batch_tiled_available(int elem, int tx, int ty, int vx, int vy);
+bool batch_copy_available(int word, int tx);
+bool batch_packed_available(int elem, int nreg);
 bool copy_shape_available(int word, int tx);
 int packed_max_nreg(int elem);
 bool packed_nreg_available(int elem, int nreg);

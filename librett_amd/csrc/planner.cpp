@@ -301,7 +301,9 @@ void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& 
     c.tile_y = shapes[s][1];
     c.vec_x = vx;
     c.vec_y = vy;
-    if (!tiled_shape_available(B, c.tile_x, c.tile_y, vx, vy)) continue;
+    if (dev.batch_mode ? !batch_tiled_available(B, c.tile_x, c.tile_y, vx, vy)
+                       : !tiled_shape_available(B, c.tile_x, c.tile_y, vx, vy))
+      continue;
     // Skip tiles more than twice the plane in a direction when a smaller one exists.
     TiledArgs& a = c.tiled;
     a.X = X;
@@ -348,6 +350,10 @@ void add_tiled_copy(const Problem& r, const DeviceInfo& dev, std::vector<Candida
   int vol_outer;
   build_outer(r, skip, cin, cout, B, W, outer, vol_outer);
   int txs[2] = {std::min(64, std::max(4, pow2ceil(X))), 64};
+  if (dev.batch_mode) {
+    txs[0] = X <= 16 ? 16 : 64;
+    txs[1] = 64;
+  }
   for (int k = 0; k < 2; k++) {
     if (k == 1 && txs[1] == txs[0]) continue;
     Candidate c;
@@ -356,7 +362,7 @@ void add_tiled_copy(const Problem& r, const DeviceInfo& dev, std::vector<Candida
     c.source = "reduced";
     c.word = W;
     c.copy_tx = txs[k];
-    if (!copy_shape_available(W, c.copy_tx)) continue;
+    if (dev.batch_mode ? !batch_copy_available(W, c.copy_tx) : !copy_shape_available(W, c.copy_tx)) continue;
     CopyArgs& a = c.copy;
     a.X = X;
     a.Y = r.dim[ydim];
@@ -390,6 +396,18 @@ void add_tiled_copy(const Problem& r, const DeviceInfo& dev, std::vector<Candida
 // Threads and registers for a staged volume: the smallest register count
 // whose thread count fits in 512, else in the device's block limit.
 bool choose_threads(int vol, int elem, const DeviceInfo& dev, int& nt, int& nreg) {
+  if (dev.batch_mode) {
+    // Fixed 256-thread workgroups; the smallest instantiated register count.
+    for (int r = 1; r <= 16; r++) {
+      if (!batch_packed_available(elem, r)) continue;
+      if (256 * r >= vol) {
+        nt = 256;
+        nreg = r;
+        return true;
+      }
+    }
+    return false;
+  }
   const int maxreg = packed_max_nreg(elem);
   for (int limit : {512, dev.max_threads_per_block}) {
     for (int r = 1; r <= maxreg; r++) {
@@ -494,6 +512,7 @@ void push_packed(const Problem& p, const char* source, const Split& s, int nt, i
     c.lds_bytes = std::max(padded_elems(a.shape[0]), padded_elems(a.shape[1])) * p.elem;
   }
   if (c.lds_bytes > dev.lds_per_block) return;
+  if (dev.batch_mode && c.lds_bytes > kBatchLds) return;
   int vol_outer;
   build_outer(p, inMmk, cin, cout, 1, 1, a.outer, vol_outer);
   a.vol_outer = vol_outer;
@@ -510,6 +529,8 @@ void push_packed(const Problem& p, const char* source, const Split& s, int nt, i
 
 void add_packed(const Problem& p, const char* source, const DeviceInfo& dev, std::vector<Candidate>& out) {
   const int B = p.elem;
+  // Staged bytes a workgroup may use (the batched kernel keeps room for padding).
+  const long long lds_cap = dev.batch_mode ? std::min(dev.lds_per_block, kBatchLds - 1024) : dev.lds_per_block;
   // Distinct staged sets over all (Mm, Mk) splits (many pairs give the same set).
   std::vector<unsigned> seen;
   for (int numMm = 1; numMm < p.rank; numMm++) {
@@ -524,7 +545,7 @@ void add_packed(const Problem& p, const char* source, const DeviceInfo& dev, std
       seen.push_back(mask);
       const long long bytes = (long long)s.volMmk * B;
       int nt, nreg;
-      if (bytes <= dev.lds_per_block && choose_threads(s.volMmk, B, dev, nt, nreg))
+      if (bytes <= lds_cap && choose_threads(s.volMmk, B, dev, nt, nreg))
         push_packed(p, source, s, nt, nreg, dev, out);
       // PackedSplit: chunk one staged dim so that a stage is about 8..128 KiB.
       // Splitting a dim that starts neither contiguous run keeps both runs
@@ -539,8 +560,8 @@ void add_packed(const Problem& p, const char* source, const DeviceInfo& dev, std
         const long long unsplit_bytes = (long long)t.volMmkUnsplit * B;
         int last_ns = -1;
         for (long long target : {8LL << 10, 16LL << 10, 32LL << 10, 64LL << 10, 128LL << 10, 1LL << 40}) {
-          const bool last = target >= dev.lds_per_block;
-          if (last) target = dev.lds_per_block;
+          const bool last = target >= lds_cap;
+          if (last) target = lds_cap;
           long long ns = (t.splitDim * unsplit_bytes + target - 1) / target;
           if (ns < 2) continue;
           if (ns > t.splitDim) ns = t.splitDim;
@@ -548,7 +569,7 @@ void add_packed(const Problem& p, const char* source, const DeviceInfo& dev, std
           ns = (t.splitDim + chunk - 1) / chunk;  // fewest splits with this chunk size
           if (ns < 2 || ns == last_ns) continue;
           if ((long long)t.splitDim * (ns + 1) >= (1LL << 31)) continue;
-          if ((long long)chunk * unsplit_bytes > dev.lds_per_block) continue;
+          if ((long long)chunk * unsplit_bytes > lds_cap) continue;
           Split u = t;
           u.numSplit = (int)ns;
           split_update(u, numMm, numMk, p);
@@ -857,6 +878,7 @@ void model_candidate(Candidate& c, const Problem& full, const DeviceInfo& dev) {
   if (c.method == kTrivial) {
     m.read_lines = m.write_lines = (double)(c.bytes / 2) / 128.0;
     m.t_hbm = (double)c.bytes / (dev.hbm_bytes_per_s * K.bw_frac);
+    m.t_stream = m.t_hbm;
     m.t_launch = K.launch;
     m.items = 1;
     m.predicted_s = m.t_hbm + m.t_launch;
@@ -885,6 +907,7 @@ void model_candidate(Candidate& c, const Problem& full, const DeviceInfo& dev) {
   const double round_bytes = (double)c.bytes / std::max(1.0, m.items);
   const double rounds = std::ceil(m.items / std::max(1.0, m.concurrency));
   m.t_hbm = std::max(std::max(t_hbm, t_cu), t_req);
+  m.t_stream = std::max(t_hbm, t_req);
   m.t_latency = rounds * (K.round_latency + round_bytes / K.block_bw);
   m.t_launch = K.launch;
   m.predicted_s = std::max(m.t_hbm, m.t_latency) + m.t_launch;

@@ -28,7 +28,13 @@ struct DeviceInfo {
   double hbm_bytes_per_s = 8.0e12;   // published MI355X HBM3E peak
   double l2_bytes = 32.0 * 1024 * 1024;  // L2 of all XCDs (8 x 4 MiB on MI355X)
   int has_gpu = 0;                   // occupancy may be queried from the runtime
+  // Batched engine: candidates must run in its persistent kernel (256-thread
+  // workgroups, at most kBatchLds bytes of LDS, its instantiated shapes).
+  int batch_mode = 0;
 };
+
+// LDS of the batched engine's persistent kernel.
+constexpr int kBatchLds = 17 * 1024;
 
 // A transpose problem: column-major input dims, output dims dim[perm[i]].
 struct Problem {
@@ -74,6 +80,7 @@ struct ModelStats {
   double concurrency = 0;     // blocks resident on the device at once
   double steps_per_block = 0; // serial load/store rounds per resident block
   double t_hbm = 0, t_latency = 0, t_launch = 0;
+  double t_stream = 0;        // bandwidth-only time (lines and requests): cost inside a busy batch
   double predicted_s = 0;
 };
 

@@ -1,0 +1,135 @@
+"""Batched engine on the GPU: every problem of a batch bit-exact against the
+oracle, for graph replays (several in a row, with other device work between
+them), plain launches, autotuned plans, unaligned buffers (fallback
+descriptors) and re-pointed buffers."""
+import numpy as np
+import pytest
+
+import librett_amd as L
+import oracle
+from librett_amd import data, hip, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_problems(n, seed, vol_hi=60000):
+    rng = np.random.default_rng(seed)
+    probs = [W.random_problem(rng, 3, 5, 1000, vol_hi, [8, 16]) for _ in range(n)]
+    probs += [([3, 4, 5], [0, 1, 2], 8), ([64, 48, 32], [2, 0, 1], 8), ([2, 8, 16, 32], [3, 1, 0, 2], 4)]
+    return probs
+
+
+class Slab:
+    """Inputs and outputs of a batch carved from two device allocations."""
+
+    def __init__(self, problems, offset=0, seed=0):
+        self.sizes = [int(np.prod(d)) * e for d, p, e in problems]
+        self.offs = []
+        o = 0
+        for s in self.sizes:
+            self.offs.append(o)
+            o += (s + 256 + 255) // 256 * 256
+        self.total = o + 512
+        self.a = hip.DeviceBuffer(self.total)
+        self.b = hip.DeviceBuffer(self.total)
+        self.offset = offset
+        self.inputs = [data.uniform(int(np.prod(d)), e, seed + i) for i, (d, p, e) in enumerate(problems)]
+        for i, x in enumerate(self.inputs):
+            hip.htod(self.a.ptr + self.offs[i] + offset, x)
+        self.refs = [oracle.transpose(d, p, e, x) for (d, p, e), x in zip(problems, self.inputs)]
+
+    def in_ptrs(self):
+        return [self.a.ptr + o + self.offset for o in self.offs]
+
+    def out_ptrs(self):
+        return [self.b.ptr + o + self.offset for o in self.offs]
+
+    def check(self):
+        bad = []
+        for i, ref in enumerate(self.refs):
+            out = np.empty_like(ref)
+            hip.dtoh(out, self.b.ptr + self.offs[i] + self.offset)
+            if not np.array_equal(out.view(np.uint8), ref.view(np.uint8)):
+                bad.append(i)
+        return bad
+
+    def free(self):
+        self.a.free()
+        self.b.free()
+
+
+def test_batch_graph_replays(gpu_stream):
+    probs = make_problems(300, 1)
+    say("case: batch of", len(probs), "problems, graph replays")
+    slab = Slab(probs)
+    b = L.Batch(probs, gpu_stream, tune=0)
+    b.set_buffers(slab.in_ptrs(), slab.out_ptrs())
+    say("  info", b.info())
+    for rep in range(3):
+        hip.memset(slab.b.ptr, 0xEE, slab.total)          # eager work between replays
+        L.device_copy(slab.b.ptr, slab.a.ptr, 1 << 20, gpu_stream, 1, 1, 0)
+        hip.stream_sync(gpu_stream)
+        hip.memset(slab.b.ptr, 0xEE, slab.total)
+        assert b.execute(use_graph=True) == L.LIBRETT_SUCCESS
+        hip.stream_sync(gpu_stream)
+        bad = slab.check()
+        assert not bad, ("replay", rep, [probs[i] for i in bad[:5]])
+    b.destroy()
+    slab.free()
+
+
+def test_batch_plain_launch_and_repoint(gpu_stream):
+    probs = make_problems(120, 2)
+    say("case: batch plain launch + re-pointed buffers")
+    s1, s2 = Slab(probs, seed=100), Slab(probs, seed=200)
+    b = L.Batch(probs, gpu_stream, tune=0)
+    for slab in (s1, s2, s1):
+        hip.memset(slab.b.ptr, 0xEE, slab.total)
+        b.set_buffers(slab.in_ptrs(), slab.out_ptrs())
+        assert b.execute(use_graph=False) == L.LIBRETT_SUCCESS
+        hip.stream_sync(gpu_stream)
+        assert not slab.check()
+        hip.memset(slab.b.ptr, 0xEE, slab.total)
+        assert b.execute(use_graph=True) == L.LIBRETT_SUCCESS
+        hip.stream_sync(gpu_stream)
+        assert not slab.check()
+    b.destroy()
+    s1.free()
+    s2.free()
+
+
+@pytest.mark.parametrize("offset", [8, 4])
+def test_batch_unaligned_buffers(gpu_stream, offset):
+    probs = [p for p in make_problems(80, 3) if p[2] >= offset]
+    say("case: batch unaligned by", offset)
+    slab = Slab(probs, offset=offset)
+    b = L.Batch(probs, gpu_stream, tune=0)
+    b.set_buffers(slab.in_ptrs(), slab.out_ptrs())
+    for _ in range(2):
+        hip.memset(slab.b.ptr, 0xEE, slab.total)
+        assert b.execute(use_graph=True) == L.LIBRETT_SUCCESS
+        hip.stream_sync(gpu_stream)
+        assert not slab.check()
+    b.destroy()
+    slab.free()
+
+
+def test_batch_autotuned(gpu_stream):
+    probs = make_problems(60, 4)
+    say("case: autotuned batch")
+    slab = Slab(probs)
+    b = L.Batch(probs, gpu_stream, tune=3)
+    info = b.info()
+    say("  info", info)
+    assert info["tuned"] >= 1
+    b.set_buffers(slab.in_ptrs(), slab.out_ptrs())
+    hip.memset(slab.b.ptr, 0xEE, slab.total)
+    assert b.execute(use_graph=True) == L.LIBRETT_SUCCESS
+    hip.stream_sync(gpu_stream)
+    assert not slab.check()
+    b.destroy()
+    slab.free()
