@@ -51,8 +51,11 @@ def main():
     ap.add_argument("--measure", action="store_true", help="choose plans with librettPlanMeasure")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=5)
-    ap.add_argument("--workload", default="large", choices=["large", "batched"],
-                    help="large: the seven ~256 MiB cases (default); batched: 4096 uneven problems")
+    ap.add_argument("--workload", default="large", choices=["large", "batched", "serving"],
+                    help="large: the seven ~256 MiB cases (default); batched: 4096 uneven problems; "
+                         "serving: Poisson request stream through the continuous-batching engine")
+    ap.add_argument("--requests", type=int, default=20000, help="serving workload: arrivals per rate")
+    ap.add_argument("--arena-gb", type=float, default=64.0, help="serving workload: device arena budget (GiB)")
     ap.add_argument("--problems", type=int, default=4096, help="batched workload: number of problems")
     ap.add_argument("--tune", type=int, default=3, help="batched workload: autotuner candidates per problem")
     args = ap.parse_args()
@@ -68,6 +71,8 @@ def main():
     stream = hip.stream_create()
     if args.workload == "batched":
         return bench_batched(args, stream)
+    if args.workload == "serving":
+        return bench_serving(args, stream)
 
     # One random source buffer, uploaded as the input of every case.
     max_bytes = max(int(np.prod(d)) * e for d, _, e in CASES)
@@ -337,6 +342,106 @@ def bench_batched(args, stream):
     A.free()
     B.free()
     print(json.dumps(line), flush=True)
+
+
+def bench_serving(args, stream):
+    """Serving workload: Poisson arrivals of requests 4 KiB-64 MiB (log-uniform),
+    rank 2-6, f32/f64/c128, random permutations, through the continuous-
+    batching engine with a device arena budget; compared with a per-call loop
+    of librettExecute over the same request mix."""
+    import librett_amd as L
+    from librett_amd import hip
+    pool_bytes = 1 << 30
+    pool = hip.DeviceBuffer(pool_bytes)
+    hip.memset(pool.ptr, 5, pool_bytes)
+    srv = L.Server(int(args.arena_gb * (1 << 30)), slots=4, max_batch=64)
+    kw = dict(catalog=1024, min_bytes=4096, max_bytes=64 << 20, rank_lo=2, rank_hi=6, elems=(4, 8, 16))
+    # Capacity: a rate far above what the device sustains (closed-loop saturation).
+    srv.run_poisson(pool.ptr, pool_bytes, qps=1e7, requests=2000, seed=1, **kw)  # warm-up
+    srv.reset_stats()
+    sat = srv.run_poisson(pool.ptr, pool_bytes, qps=1e7, requests=args.requests, seed=2, **kw)
+    mean_req_bytes = sat["bytes"] / max(1, sat["completed"])
+    cap_qps = sat["gbs"] * 1e9 / mean_req_bytes
+    log("saturation: %.1f GB/s, %.0f req/s, mean request %.2f MB moved, mean batch %.1f" % (
+        sat["gbs"], cap_qps, mean_req_bytes / 1e6, sat["mean_batch"]))
+    rates = []
+    for frac in (0.25, 0.5, 0.75, 0.9):
+        srv.reset_stats()
+        st = srv.run_poisson(pool.ptr, pool_bytes, qps=frac * cap_qps, requests=args.requests, seed=3, **kw)
+        rates.append(dict(load=frac, qps=round(frac * cap_qps), gbs=round(st["gbs"], 1), p50_us=round(st["p50_us"], 1),
+                          p90_us=round(st["p90_us"], 1), p99_us=round(st["p99_us"], 1),
+                          mean_batch=round(st["mean_batch"], 2), stalls=st["admission_stalls"],
+                          arena_peak_gb=round(st["arena_peak_bytes"] / 1e9, 2)))
+        log("load %.2f: %8.0f req/s  %7.1f GB/s  p50 %8.1f us  p99 %8.1f us  batch %.1f  stalls %d" % (
+            frac, frac * cap_qps, st["gbs"], st["p50_us"], st["p99_us"], st["mean_batch"], st["admission_stalls"]))
+    srv.destroy()
+    # Per-call baseline: the same catalog through librettExecute, one stream,
+    # plans built beforehand, back to back.
+    import ctypes
+    cat = serving_catalog(kw, 1024)
+    handles = []
+    for d, p, e in cat:
+        r, h = L.librettPlan(len(d), list(d), list(p), e, stream)
+        handles.append((h, int(np.prod(d)) * e))
+    out = hip.DeviceBuffer(64 << 20)
+    rng = np.random.default_rng(4)
+    picks = rng.integers(0, len(handles), size=args.requests)
+    offs = rng.integers(0, (pool_bytes - (64 << 20)) // 256, size=args.requests) * 256
+    for i in range(200):
+        h, nb = handles[picks[i]]
+        L.librettExecute(h, pool.ptr + int(offs[i]), out.ptr)
+    hip.stream_sync(stream)
+    t0 = time.perf_counter()
+    moved = 0
+    for i in range(args.requests):
+        h, nb = handles[picks[i]]
+        L.librettExecute(h, pool.ptr + int(offs[i]), out.ptr)
+        moved += 2 * nb
+    hip.stream_sync(stream)
+    loop_gbs = moved / (time.perf_counter() - t0) / 1e9
+    log("per-call loop: %.1f GB/s" % loop_gbs)
+    for h, _ in handles:
+        L.librettDestroy(h)
+    out.free()
+    pool.free()
+    best = max(rates, key=lambda r: r["gbs"])
+    line = {
+        "metric": "serving transpose GB/s and p50/p99 latency (Poisson arrivals, continuous batching)",
+        "value": round(sat["gbs"], 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.requests,
+        "warmup": 2000,
+        "ms_per_step": round(sat["wall_seconds"] / max(1, sat["completed"]) * 1e3, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64/c128 (opaque moves)",
+        "data": "synthetic",
+        "config": {"workload": "serving: Poisson arrivals, 4 KiB-64 MiB log-uniform, rank 2-6, f32/f64/c128, "
+                               "%g GiB arena, LRU plan cache, catalog of 1024 shapes" % args.arena_gb},
+        "saturation": {"gbs": round(sat["gbs"], 1), "req_per_s": round(cap_qps), "p50_us": round(sat["p50_us"], 1),
+                       "p99_us": round(sat["p99_us"], 1), "mean_batch": round(sat["mean_batch"], 2)},
+        "load_sweep": rates,
+        "per_call_loop_gbs": round(loop_gbs, 1),
+        "speedup_vs_per_call": round(sat["gbs"] / loop_gbs, 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def serving_catalog(kw, n):
+    """Host copy of the request shapes the serving engine draws (for the
+    per-call baseline): same distribution, numpy generator."""
+    from librett_amd import workloads as W
+    rng = np.random.default_rng(kw.get("seed", 77))
+    out = []
+    for _ in range(n):
+        e = int(rng.choice(kw["elems"]))
+        nbytes = float(np.exp(rng.uniform(np.log(kw["min_bytes"]), np.log(kw["max_bytes"]))))
+        rank = int(rng.integers(kw["rank_lo"], kw["rank_hi"] + 1))
+        d, p, _ = W.random_problem(rng, rank, rank, max(2.0, nbytes / e), max(2.0, nbytes / e), [e])
+        out.append((d, p, e))
+    return out
 
 
 def cpu_baseline(reps):

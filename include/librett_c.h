@@ -170,6 +170,62 @@ int librett_c_batch_info_get(unsigned int handle, librett_c_batch_info* info);
 int librett_c_batch_host_replay(int count, const int* ranks, const int* dims, const int* perms, const size_t* elems,
                                 const librett_c_device* dev, const void* const* idata, void* const* odata, int grid);
 
+/* ---------------------------------------------------------------------
+ * Serving engine (extension; the reference has no serving mode).
+ * Continuous batching: requests queued from any thread are launched as
+ * micro-batches of the batched engine on `slots` streams as soon as a slot
+ * frees up, so requests arriving while others run overlap with them on the
+ * device. Output buffers come from a device arena with a hard byte budget;
+ * plans are cached in an LRU. Execution order between requests is not
+ * defined; each request's result does not depend on the others.
+ * ------------------------------------------------------------------- */
+#define LIBRETT_C_BUSY 6 /* the arena cannot hold the allocation right now */
+
+typedef struct librett_c_server_stats {
+  long long submitted, completed, failed;
+  long long bytes;              /* algorithmic bytes of completed requests */
+  long long batches;            /* micro-batches launched */
+  double mean_batch;            /* requests per micro-batch */
+  long long admission_stalls;   /* arrivals that waited for arena space */
+  double stall_seconds;
+  long long arena_peak_bytes, arena_in_use_bytes;
+  double p50_us, p90_us, p99_us, max_us, mean_us; /* arrival -> completion */
+  double wall_seconds;          /* first arrival -> last completion */
+  double gbs;                   /* bytes / wall_seconds */
+} librett_c_server_stats;
+
+typedef struct librett_c_poisson {
+  double qps;                 /* mean arrival rate */
+  int requests;               /* number of arrivals */
+  unsigned long long seed;
+  int catalog;                /* distinct request shapes, drawn up front */
+  double min_bytes, max_bytes;/* tensor bytes, log-uniform */
+  int rank_lo, rank_hi;
+  int elems[3];               /* element sizes drawn uniformly */
+  int nelems;
+} librett_c_poisson;
+
+int librett_c_server_create(unsigned int* handle, size_t arena_bytes, int slots, int max_batch);
+/* Arena allocation (256-byte aligned); LIBRETT_C_BUSY when it does not fit now. */
+int librett_c_server_alloc(unsigned int handle, size_t bytes, void** ptr);
+int librett_c_server_free(unsigned int handle, void* ptr);
+/* Queues one request; returns at once with a ticket. */
+int librett_c_server_submit(unsigned int handle, int rank, const int* dim, const int* permutation,
+                            size_t sizeofType, const void* idata, void* odata, unsigned long long* ticket);
+/* Blocks until the request is complete; latency = completion - submission. */
+int librett_c_server_wait(unsigned int handle, unsigned long long ticket, double* latency_s);
+int librett_c_server_drain(unsigned int handle);
+int librett_c_server_stats_get(unsigned int handle, librett_c_server_stats* stats);
+int librett_c_server_reset_stats(unsigned int handle);
+int librett_c_server_destroy(unsigned int handle);
+/* Native Poisson load generator: `requests` arrivals at `qps`, shapes from a
+ * catalog, inputs read from slices of `input_pool` (device memory of
+ * pool_bytes >= max_bytes), outputs allocated from the arena at arrival
+ * (waiting while it is full) and returned at completion. Blocks until all
+ * requests complete. */
+int librett_c_server_run_poisson(unsigned int handle, const librett_c_poisson* cfg, const void* input_pool,
+                                 size_t pool_bytes);
+
 /* Extension: library version string. */
 const char* librett_c_version(void);
 

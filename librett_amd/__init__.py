@@ -73,7 +73,34 @@ EXPORTED = [
     "librett_c_reduce_ranks", "librett_c_device_copy", "librett_c_set_model_constants", "librett_c_version",
     "librett_c_batch_plan", "librett_c_batch_set_buffers", "librett_c_batch_execute", "librett_c_batch_destroy",
     "librett_c_batch_info_get", "librett_c_batch_host_replay",
+    "librett_c_server_create", "librett_c_server_alloc", "librett_c_server_free", "librett_c_server_submit",
+    "librett_c_server_wait", "librett_c_server_drain", "librett_c_server_stats_get", "librett_c_server_reset_stats",
+    "librett_c_server_destroy", "librett_c_server_run_poisson",
 ]
+LIBRETT_BUSY = 6
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong),
+        ("bytes", ctypes.c_longlong), ("batches", ctypes.c_longlong), ("mean_batch", ctypes.c_double),
+        ("admission_stalls", ctypes.c_longlong), ("stall_seconds", ctypes.c_double),
+        ("arena_peak_bytes", ctypes.c_longlong), ("arena_in_use_bytes", ctypes.c_longlong),
+        ("p50_us", ctypes.c_double), ("p90_us", ctypes.c_double), ("p99_us", ctypes.c_double),
+        ("max_us", ctypes.c_double), ("mean_us", ctypes.c_double), ("wall_seconds", ctypes.c_double),
+        ("gbs", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class PoissonConfig(ctypes.Structure):
+    _fields_ = [
+        ("qps", ctypes.c_double), ("requests", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("catalog", ctypes.c_int),
+        ("min_bytes", ctypes.c_double), ("max_bytes", ctypes.c_double), ("rank_lo", ctypes.c_int),
+        ("rank_hi", ctypes.c_int), ("elems", ctypes.c_int * 3), ("nelems", ctypes.c_int),
+    ]
 
 
 class BatchInfo(ctypes.Structure):
@@ -126,6 +153,17 @@ def lib():
         L.librett_c_batch_info_get.argtypes = [ctypes.c_uint, ctypes.POINTER(BatchInfo)]
         L.librett_c_batch_host_replay.argtypes = [ctypes.c_int, ip, ip, ip, szp, ctypes.POINTER(DeviceDesc), pp, pp,
                                                   ctypes.c_int]
+        ullp = ctypes.POINTER(ctypes.c_ulonglong)
+        L.librett_c_server_create.argtypes = [up, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+        L.librett_c_server_alloc.argtypes = [ctypes.c_uint, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+        L.librett_c_server_free.argtypes = [ctypes.c_uint, vp]
+        L.librett_c_server_submit.argtypes = [ctypes.c_uint, ctypes.c_int, ip, ip, ctypes.c_size_t, vp, vp, ullp]
+        L.librett_c_server_wait.argtypes = [ctypes.c_uint, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_double)]
+        L.librett_c_server_drain.argtypes = [ctypes.c_uint]
+        L.librett_c_server_stats_get.argtypes = [ctypes.c_uint, ctypes.POINTER(ServerStats)]
+        L.librett_c_server_reset_stats.argtypes = [ctypes.c_uint]
+        L.librett_c_server_destroy.argtypes = [ctypes.c_uint]
+        L.librett_c_server_run_poisson.argtypes = [ctypes.c_uint, ctypes.POINTER(PoissonConfig), vp, ctypes.c_size_t]
         L.librett_c_version.restype = ctypes.c_char_p
         for name in EXPORTED:
             f = getattr(L, name)
@@ -299,6 +337,72 @@ class Batch:
     def destroy(self):
         if self.handle:
             lib().librett_c_batch_destroy(self.handle)
+            self.handle = 0
+
+
+class Server:
+    """Continuous-batching serving engine (extension of the API)."""
+
+    def __init__(self, arena_bytes, slots=4, max_batch=64):
+        h = ctypes.c_uint(0)
+        r = lib().librett_c_server_create(ctypes.byref(h), int(arena_bytes), int(slots), int(max_batch))
+        if r != LIBRETT_SUCCESS:
+            raise ValueError("server create failed: %d" % r)
+        self.handle = h.value
+
+    def alloc(self, nbytes):
+        """Device pointer from the arena, or None when it is full."""
+        p = ctypes.c_void_p()
+        r = lib().librett_c_server_alloc(self.handle, int(nbytes), ctypes.byref(p))
+        if r == LIBRETT_BUSY:
+            return None
+        if r != LIBRETT_SUCCESS:
+            raise ValueError("server alloc failed: %d" % r)
+        return p.value
+
+    def free(self, ptr):
+        return lib().librett_c_server_free(self.handle, ctypes.c_void_p(ptr))
+
+    def submit(self, dim, perm, elem, idata, odata):
+        t = ctypes.c_ulonglong(0)
+        r = lib().librett_c_server_submit(self.handle, len(dim), _iarr(dim), _iarr(perm), int(elem),
+                                          ctypes.c_void_p(idata), ctypes.c_void_p(odata), ctypes.byref(t))
+        if r != LIBRETT_SUCCESS:
+            raise ValueError("server submit failed: %d" % r)
+        return t.value
+
+    def wait(self, ticket):
+        lat = ctypes.c_double(0)
+        r = lib().librett_c_server_wait(self.handle, ctypes.c_ulonglong(ticket), ctypes.byref(lat))
+        if r != LIBRETT_SUCCESS:
+            raise ValueError("server wait failed: %d" % r)
+        return lat.value
+
+    def drain(self):
+        lib().librett_c_server_drain(self.handle)
+
+    def stats(self):
+        st = ServerStats()
+        lib().librett_c_server_stats_get(self.handle, ctypes.byref(st))
+        return st.as_dict()
+
+    def reset_stats(self):
+        lib().librett_c_server_reset_stats(self.handle)
+
+    def run_poisson(self, pool_ptr, pool_bytes, qps, requests, seed=20261202, catalog=1024, min_bytes=4096,
+                    max_bytes=64 << 20, rank_lo=2, rank_hi=6, elems=(4, 8, 16)):
+        cfg = PoissonConfig(float(qps), int(requests), int(seed), int(catalog), float(min_bytes), float(max_bytes),
+                            int(rank_lo), int(rank_hi), (ctypes.c_int * 3)(*(list(elems) + [4, 4, 4])[:3]),
+                            len(elems))
+        r = lib().librett_c_server_run_poisson(self.handle, ctypes.byref(cfg), ctypes.c_void_p(pool_ptr),
+                                               int(pool_bytes))
+        if r != LIBRETT_SUCCESS:
+            raise ValueError("run_poisson failed: %d" % r)
+        return self.stats()
+
+    def destroy(self):
+        if self.handle:
+            lib().librett_c_server_destroy(self.handle)
             self.handle = 0
 
 

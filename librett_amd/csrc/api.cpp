@@ -231,7 +231,7 @@ double time_candidate(const Candidate& c, hipStream_t stream, const void* in, vo
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return -1;
   if (hipEventCreate(&e1) != hipSuccess) {
-    hipEventDestroy(e0);
+    (void)hipEventDestroy(e0);
     return -1;
   }
   std::vector<float> ts;
@@ -247,8 +247,8 @@ double time_candidate(const Candidate& c, hipStream_t stream, const void* in, vo
     ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
     if (ok) ts.push_back(ms);
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (!ok || ts.empty()) {
     (void)hipGetLastError();
     return -1;

@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "planner.h"
 
 namespace lrt {
@@ -44,6 +46,19 @@ hipError_t launch_batch(const BatchDesc* d_descs, const int* d_start, int nprob,
 
 // Resident workgroups per CU of the persistent kernel (runtime query).
 int batch_occupancy();
+
+// Descriptors of one problem planned with candidate c: one descriptor, or
+// up to two for a plain copy (Trivial). `fallback` gets the variants for
+// buffers not aligned to the vector width; `offsets` the byte offset of each
+// descriptor's data inside the problem's buffers.
+bool describe_problem(const Problem& p, const Candidate& c, std::vector<BatchDesc>& descs,
+                      std::vector<BatchDesc>& fallback, std::vector<long long>& offsets);
+
+// Pointer alignment the descriptors of candidate c need.
+int batch_alignment(const Problem& p, const Candidate& c);
+
+// Batch-mode candidates of a problem, best modelled streaming cost first.
+std::vector<Candidate> batch_candidates(const Problem& p, const DeviceInfo& dev);
 
 // Host replay of the persistent kernel (descriptors hold host pointers).
 void replay_batch(const BatchDesc* descs, const int* start, int nprob, int total, int grid);

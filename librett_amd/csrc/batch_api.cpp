@@ -127,23 +127,6 @@ std::string problem_key(const Problem& p) {
   return k;
 }
 
-int desc_alignment(const Candidate& c) {
-  if (c.method == kTiled) return c.elem * std::max(c.vec_x, c.vec_y);
-  if (c.method == kTiledCopy) return c.word;
-  return c.elem;
-}
-
-// Batch-mode candidates of a problem, best streaming cost first.
-std::vector<Candidate> ranked_candidates(const Problem& p, const DeviceInfo& dev) {
-  std::vector<Candidate> cands = make_candidates(p, dev);
-  std::vector<Candidate> ok;
-  for (auto& c : cands)
-    if (c.method == kTrivial || batch_variant(c) >= 0) ok.push_back(c);
-  std::stable_sort(ok.begin(), ok.end(), [](const Candidate& a, const Candidate& b) {
-    return a.model.t_stream < b.model.t_stream;
-  });
-  return ok;
-}
 
 // Times candidate c of problem p inside the persistent kernel, on enough
 // clones of the problem to fill the device (scratch buffers `a`, `b`).
@@ -198,7 +181,7 @@ int build_batch(BatchObj& B, const DeviceInfo& dev, int tune) {
   B.chosen.resize(B.nprob);
   std::vector<std::vector<Candidate>> ranked(B.nprob);
   for (int i = 0; i < B.nprob; i++) {
-    ranked[i] = ranked_candidates(B.probs[i], bdev);
+    ranked[i] = batch_candidates(B.probs[i], bdev);
     if (ranked[i].empty()) return LIBRETT_C_INTERNAL_ERROR;
     B.chosen[i] = ranked[i][0];
   }
@@ -259,54 +242,15 @@ int build_batch(BatchObj& B, const DeviceInfo& dev, int tune) {
   B.owner.clear();
   B.byte_offset.clear();
   for (int i = 0; i < B.nprob; i++) {
-    const Candidate& c = B.chosen[i];
-    if (c.method == kTrivial) {
-      // Element-sized words: always aligned, so the fallback is the same.
-      std::vector<BatchDesc> ds;
-      std::vector<long long> offs;
-      copy_descs((long long)B.probs[i].vol * B.probs[i].elem, B.probs[i].elem, ds, offs);
-      for (size_t k = 0; k < ds.size(); k++) {
-        B.descs.push_back(ds[k]);
-        B.fallback.push_back(ds[k]);
-        B.owner.push_back(i);
-        B.byte_offset.push_back(offs[k]);
-      }
-      continue;
+    std::vector<BatchDesc> ds, fs;
+    std::vector<long long> offs;
+    if (!describe_problem(B.probs[i], B.chosen[i], ds, fs, offs)) return LIBRETT_C_INTERNAL_ERROR;
+    for (size_t k = 0; k < ds.size(); k++) {
+      B.descs.push_back(ds[k]);
+      B.fallback.push_back(fs[k]);
+      B.owner.push_back(i);
+      B.byte_offset.push_back(offs[k]);
     }
-    BatchDesc d, f;
-    if (!batch_describe(c, d)) return LIBRETT_C_INTERNAL_ERROR;
-    Candidate fc = c;
-    if (c.method == kTiled) {
-      fc.vec_x = fc.vec_y = 1;
-    }
-    if (!batch_describe(fc, f)) f = d;
-    if (c.method == kTiledCopy && c.word > c.elem) {
-      // Element-sized words for unaligned buffers.
-      Candidate e = c;
-      const int k = c.word / c.elem;
-      CopyArgs& a = e.copy;
-      a.X *= k;
-      a.in_ys *= k;
-      a.out_ys *= k;
-      for (int r = 0; r < a.outer.n; r++) {
-        a.outer.ct_in[r] *= k;
-        a.outer.ct_out[r] *= k;
-      }
-      e.word = c.elem;
-      e.copy_tx = 64;
-      const int ty = (256 / e.copy_tx) * 4;
-      const long long vol_outer = c.copy.items / std::max(1, c.copy.ntiles);
-      a.ntx = (a.X + e.copy_tx - 1) / e.copy_tx;
-      a.ntiles = a.ntx * ((a.Y + ty - 1) / ty);
-      a.ntx_div = make_fastdiv((uint32_t)a.ntx);
-      a.ntiles_div = make_fastdiv((uint32_t)a.ntiles);
-      a.items = (int)(a.ntiles * vol_outer);
-      if (!batch_describe(e, f)) f = d;
-    }
-    B.descs.push_back(d);
-    B.fallback.push_back(f);
-    B.owner.push_back(i);
-    B.byte_offset.push_back(0);
   }
   const int nd = (int)B.descs.size();
   B.start.assign(nd + 1, 0);
@@ -365,6 +309,70 @@ int read_problems(int count, const int* ranks, const int* dims, const int* perms
 }
 
 }  // namespace
+
+std::vector<Candidate> batch_candidates(const Problem& p, const DeviceInfo& dev) {
+  DeviceInfo bdev = dev;
+  bdev.batch_mode = 1;
+  std::vector<Candidate> cands = make_candidates(p, bdev);
+  std::vector<Candidate> ok;
+  for (auto& c : cands)
+    if (c.method == kTrivial || batch_variant(c) >= 0) ok.push_back(c);
+  std::stable_sort(ok.begin(), ok.end(), [](const Candidate& a, const Candidate& b) {
+    return a.model.t_stream < b.model.t_stream;
+  });
+  return ok;
+}
+
+int batch_alignment(const Problem& p, const Candidate& c) {
+  if (c.method == kTrivial) return p.elem;
+  if (c.method == kTiled) return c.elem * std::max(c.vec_x, c.vec_y);
+  if (c.method == kTiledCopy) return c.word;
+  return c.elem;
+}
+
+bool describe_problem(const Problem& p, const Candidate& c, std::vector<BatchDesc>& descs,
+                      std::vector<BatchDesc>& fallback, std::vector<long long>& offsets) {
+  descs.clear();
+  fallback.clear();
+  offsets.clear();
+  if (c.method == kTrivial) {
+    // Element-sized words: always aligned, so the fallback is the same.
+    copy_descs((long long)p.vol * p.elem, p.elem, descs, offsets);
+    fallback = descs;
+    return !descs.empty();
+  }
+  BatchDesc d, f;
+  if (!batch_describe(c, d)) return false;
+  Candidate fc = c;
+  if (c.method == kTiled) fc.vec_x = fc.vec_y = 1;
+  if (c.method == kTiledCopy && c.word > c.elem) {
+    // Element-sized words for unaligned buffers.
+    const int k = c.word / c.elem;
+    CopyArgs& a = fc.copy;
+    a.X *= k;
+    a.in_ys *= k;
+    a.out_ys *= k;
+    for (int r = 0; r < a.outer.n; r++) {
+      a.outer.ct_in[r] *= k;
+      a.outer.ct_out[r] *= k;
+    }
+    fc.word = c.elem;
+    fc.copy_tx = 64;
+    const int ty = (256 / fc.copy_tx) * 4;
+    const long long vol_outer = c.copy.items / std::max(1, c.copy.ntiles);
+    a.ntx = (a.X + fc.copy_tx - 1) / fc.copy_tx;
+    a.ntiles = a.ntx * ((a.Y + ty - 1) / ty);
+    a.ntx_div = make_fastdiv((uint32_t)a.ntx);
+    a.ntiles_div = make_fastdiv((uint32_t)a.ntiles);
+    a.items = (int)(a.ntiles * vol_outer);
+  }
+  if (!batch_describe(fc, f)) f = d;
+  descs.push_back(d);
+  fallback.push_back(f);
+  offsets.push_back(0);
+  return true;
+}
+
 }  // namespace lrt
 
 using namespace lrt;
@@ -415,8 +423,7 @@ int librett_c_batch_set_buffers(unsigned int handle, const void* const* idata, v
     if (!idata[i] || !odata[i] || idata[i] == odata[i]) return LIBRETT_C_INVALID_PARAMETER;
     const char* in = (const char*)idata[i] + B->byte_offset[k];
     char* out = (char*)odata[i] + B->byte_offset[k];
-    const Candidate& c = B->chosen[i];
-    const int align = c.method == kTrivial ? B->probs[i].elem : desc_alignment(c);
+    const int align = batch_alignment(B->probs[i], B->chosen[i]);
     const bool mis = (((uintptr_t)in | (uintptr_t)out) & (uintptr_t)(align - 1)) != 0;
     up[k] = mis ? B->fallback[k] : B->descs[k];
     up[k].in = in;
