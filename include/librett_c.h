@@ -170,6 +170,19 @@ int librett_c_batch_info_get(unsigned int handle, librett_c_batch_info* info);
 int librett_c_batch_host_replay(int count, const int* ranks, const int* dims, const int* perms, const size_t* elems,
                                 const librett_c_device* dev, const void* const* idata, void* const* odata, int grid);
 
+/* Host replay of batch-mode candidate `index` of one problem (the order of
+ * the batch planner's ranking), through the persistent kernel's per-thread
+ * code between guard bands. `fallback` selects the descriptor for buffers
+ * not aligned to the vector width; `unit_bytes` the work-unit size (0:
+ * default); `grid` the workgroups replayed (0: 4 per CU). With idata and
+ * odata both NULL it only stores the number of candidates in *count.
+ * Returns LIBRETT_UNDEFINED_ERROR if a store leaves the output or a unit
+ * writes past the kernel's LDS. No reference counterpart: test support for
+ * the autotuner, which runs candidates other than the model's choice. */
+int librett_c_batch_host_candidate(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                                   const librett_c_device* device, int index, int fallback, long long unit_bytes,
+                                   int grid, const void* idata, void* odata, int* count);
+
 /* ---------------------------------------------------------------------
  * Serving engine (extension; the reference has no serving mode).
  * Continuous batching: requests queued from any thread are launched as

@@ -72,7 +72,7 @@ EXPORTED = [
     "librett_c_plan_candidate", "librett_c_host_candidates", "librett_c_host_replay",
     "librett_c_reduce_ranks", "librett_c_device_copy", "librett_c_set_model_constants", "librett_c_version",
     "librett_c_batch_plan", "librett_c_batch_set_buffers", "librett_c_batch_execute", "librett_c_batch_destroy",
-    "librett_c_batch_info_get", "librett_c_batch_host_replay",
+    "librett_c_batch_info_get", "librett_c_batch_host_replay", "librett_c_batch_host_candidate",
     "librett_c_server_create", "librett_c_server_alloc", "librett_c_server_free", "librett_c_server_submit",
     "librett_c_server_wait", "librett_c_server_drain", "librett_c_server_stats_get", "librett_c_server_reset_stats",
     "librett_c_server_destroy", "librett_c_server_run_poisson",
@@ -153,6 +153,9 @@ def lib():
         L.librett_c_batch_info_get.argtypes = [ctypes.c_uint, ctypes.POINTER(BatchInfo)]
         L.librett_c_batch_host_replay.argtypes = [ctypes.c_int, ip, ip, ip, szp, ctypes.POINTER(DeviceDesc), pp, pp,
                                                   ctypes.c_int]
+        L.librett_c_batch_host_candidate.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, ctypes.POINTER(DeviceDesc),
+                                                     ctypes.c_int, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, vp,
+                                                     vp, ip]
         ullp = ctypes.POINTER(ctypes.c_ulonglong)
         L.librett_c_server_create.argtypes = [up, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
         L.librett_c_server_alloc.argtypes = [ctypes.c_uint, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
@@ -418,6 +421,30 @@ def batch_host_replay(problems, inputs, grid=0, device=None):
     if r != LIBRETT_SUCCESS:
         raise ValueError("batch host replay failed: %d" % r)
     return outs
+
+
+def batch_host_candidates(dim, permutation, sizeofType, device=None):
+    """Number of batch-mode candidates of one problem."""
+    n = ctypes.c_int(0)
+    r = lib().librett_c_batch_host_candidate(len(dim), _iarr(dim), _iarr(permutation), int(sizeofType),
+                                             ctypes.byref(device or mi355x_desc()), 0, 0, 0, 0, None, None,
+                                             ctypes.byref(n))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("batch candidates failed: %d" % r)
+    return n.value
+
+
+def batch_host_replay_candidate(dim, permutation, sizeofType, x, index, fallback=False, unit_bytes=0, grid=0,
+                                device=None):
+    """Host replay of batch-mode candidate `index`; returns (status, output)."""
+    import numpy as np
+    x = np.ascontiguousarray(x)
+    y = np.zeros_like(x)
+    n = ctypes.c_int(0)
+    r = lib().librett_c_batch_host_candidate(len(dim), _iarr(dim), _iarr(permutation), int(sizeofType),
+                                             ctypes.byref(device or mi355x_desc()), int(index), int(bool(fallback)),
+                                             int(unit_bytes), int(grid), x.ctypes.data, y.ctypes.data, ctypes.byref(n))
+    return r, y
 
 
 def version():

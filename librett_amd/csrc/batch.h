@@ -21,8 +21,10 @@ namespace lrt {
 struct BatchDesc {
   int variant;       // index into the persistent kernel's shape table
   int units;         // work units of this problem
-  int rounds;        // Packed: outer indices per unit
-  int pad_;
+  int rounds;        // items per unit (at most)
+  int items;         // tiles (Tiled, TiledCopy) or outer indices (Packed) of the problem
+  int groups;        // unit g of a split chunk takes items g, g + groups, g + 2 groups, ...
+  int item_bytes;    // algorithmic bytes (read + write) of one item
   FastDiv split_div; // Packed: number of chunks of the split dim
   const void* in;
   void* out;
@@ -36,8 +38,21 @@ struct BatchDesc {
 // Shape-table index of a batch-mode candidate, or -1.
 int batch_variant(const Candidate& c);
 
+// Default algorithmic bytes of one work unit.
+constexpr long long kBatchUnitBytes = 32 << 10;
+
 // Fills d (except pointers) for a candidate; returns false if unsupported.
 bool batch_describe(const Candidate& c, BatchDesc& d);
+
+// Regroups d's items into units of about `unit_bytes` algorithmic bytes
+// (at least one item per unit).
+void batch_regroup(BatchDesc& d, long long unit_bytes);
+
+// Unit size for a batch of `total_bytes` algorithmic bytes run by `grid`
+// workgroups: about 32 units per workgroup, between 16 KiB and
+// kBatchUnitBytes (environment overrides: LIBRETT_BATCH_UNIT_KB,
+// LIBRETT_BATCH_UNITS_PER_WG).
+long long batch_unit_bytes(long long total_bytes, int grid);
 
 // Launches the persistent kernel over `total` units of `nprob` problems.
 // `start` holds the first unit of every problem plus `total` at the end.
@@ -61,6 +76,7 @@ int batch_alignment(const Problem& p, const Candidate& c);
 std::vector<Candidate> batch_candidates(const Problem& p, const DeviceInfo& dev);
 
 // Host replay of the persistent kernel (descriptors hold host pointers).
-void replay_batch(const BatchDesc* descs, const int* start, int nprob, int total, int grid);
+// Returns false if a unit wrote past the kernel's LDS.
+bool replay_batch(const BatchDesc* descs, const int* start, int nprob, int total, int grid);
 
 }  // namespace lrt

@@ -5,6 +5,8 @@
 #include <vector>
 
 #include "batch.h"
+
+#include <cstdlib>
 #include "kernel_ops.h"
 #include "kernels.h"
 
@@ -56,19 +58,23 @@ __device__ __forceinline__ void dev_tiled(const BatchDesc& D, int lu, unsigned c
   using Ops = TiledOps<B, TX, TY, VX, VY>;
   using E = typename Ops::E;
   E* lds = reinterpret_cast<E*>(lds_raw);
-  const TileCtx c = Ops::item(D.a.tiled, (uint32_t)lu);
-  Ops::load(D.a.tiled, c, threadIdx.x, static_cast<const E*>(D.in), lds);
-  __syncthreads();
-  Ops::store(D.a.tiled, c, threadIdx.x, static_cast<E*>(D.out), lds);
-  __syncthreads();
+  for (int w = lu; w < D.items; w += D.groups) {
+    const TileCtx c = Ops::item(D.a.tiled, (uint32_t)w);
+    Ops::load(D.a.tiled, c, threadIdx.x, static_cast<const E*>(D.in), lds);
+    __syncthreads();
+    Ops::store(D.a.tiled, c, threadIdx.x, static_cast<E*>(D.out), lds);
+    __syncthreads();
+  }
 }
 
 template <int W, int TX>
 __device__ __forceinline__ void dev_copy(const BatchDesc& D, int lu) {
   using Ops = CopyOps<W, TX, 4>;
   using V = typename Ops::V;
-  const TileCtx c = Ops::item(D.a.copy, (uint32_t)lu);
-  Ops::run(D.a.copy, c, threadIdx.x, static_cast<const V*>(D.in), static_cast<V*>(D.out));
+  for (int w = lu; w < D.items; w += D.groups) {
+    const TileCtx c = Ops::item(D.a.copy, (uint32_t)w);
+    Ops::run(D.a.copy, c, threadIdx.x, static_cast<const V*>(D.in), static_cast<V*>(D.out));
+  }
 }
 
 template <int B, int NREG>
@@ -81,9 +87,7 @@ __device__ __forceinline__ void dev_packed(const BatchDesc& D, int lu, unsigned 
   const int split = lu - (int)(og * D.split_div.d);
   typename Ops::Pre p;
   Ops::prepare(a, split, threadIdx.x, 256, p);
-  const int o0 = (int)og * D.rounds;
-  const int o1 = min(a.vol_outer, o0 + D.rounds);
-  for (int o = o0; o < o1; o++) {
+  for (int o = (int)og; o < a.vol_outer; o += D.groups) {
     int off_in, off_out;
     outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
     Ops::load(p, off_in, threadIdx.x, 256, static_cast<const E*>(D.in), lds);
@@ -144,17 +148,21 @@ void host_tiled(const BatchDesc& D, int lu, unsigned char* lds_raw) {
   using Ops = TiledOps<B, TX, TY, VX, VY>;
   using E = typename Ops::E;
   E* lds = reinterpret_cast<E*>(lds_raw);
-  const TileCtx c = Ops::item(D.a.tiled, (uint32_t)lu);
-  for (int t = 0; t < 256; t++) Ops::load(D.a.tiled, c, t, static_cast<const E*>(D.in), lds);
-  for (int t = 0; t < 256; t++) Ops::store(D.a.tiled, c, t, static_cast<E*>(D.out), lds);
+  for (int w = lu; w < D.items; w += D.groups) {
+    const TileCtx c = Ops::item(D.a.tiled, (uint32_t)w);
+    for (int t = 0; t < 256; t++) Ops::load(D.a.tiled, c, t, static_cast<const E*>(D.in), lds);
+    for (int t = 0; t < 256; t++) Ops::store(D.a.tiled, c, t, static_cast<E*>(D.out), lds);
+  }
 }
 
 template <int W, int TX>
 void host_copy(const BatchDesc& D, int lu) {
   using Ops = CopyOps<W, TX, 4>;
   using V = typename Ops::V;
-  const TileCtx c = Ops::item(D.a.copy, (uint32_t)lu);
-  for (int t = 0; t < 256; t++) Ops::run(D.a.copy, c, t, static_cast<const V*>(D.in), static_cast<V*>(D.out));
+  for (int w = lu; w < D.items; w += D.groups) {
+    const TileCtx c = Ops::item(D.a.copy, (uint32_t)w);
+    for (int t = 0; t < 256; t++) Ops::run(D.a.copy, c, t, static_cast<const V*>(D.in), static_cast<V*>(D.out));
+  }
 }
 
 template <int B, int NREG>
@@ -167,9 +175,7 @@ void host_packed(const BatchDesc& D, int lu, unsigned char* lds_raw) {
   const int split = lu - (int)(og * D.split_div.d);
   std::vector<typename Ops::Pre> pre(256);
   for (int t = 0; t < 256; t++) Ops::prepare(a, split, t, 256, pre[t]);
-  const int o0 = (int)og * D.rounds;
-  const int o1 = std::min(a.vol_outer, o0 + D.rounds);
-  for (int o = o0; o < o1; o++) {
+  for (int o = (int)og; o < a.vol_outer; o += D.groups) {
     int off_in, off_out;
     outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
     for (int t = 0; t < 256; t++) Ops::load(pre[t], off_in, t, 256, static_cast<const E*>(D.in), lds);
@@ -255,30 +261,52 @@ bool batch_describe(const Candidate& c, BatchDesc& d) {
   d = BatchDesc();
   d.variant = batch_variant(c);
   if (d.variant < 0) return false;
+  if (c.lds_bytes > kBatchLds) return false;
   d.split_div = make_fastdiv(1);
-  d.rounds = 1;
   switch (c.method) {
     case kTiled:
       d.a.tiled = c.tiled;
-      d.units = c.tiled.items;
+      d.items = c.tiled.items;
+      d.item_bytes = 2 * c.tile_x * c.tile_y * c.elem;
       break;
     case kTiledCopy:
       d.a.copy = c.copy;
-      d.units = c.copy.items;
+      d.items = c.copy.items;
+      d.item_bytes = 2 * 1024 * c.word;  // copy_tx words x 1024/copy_tx rows
       break;
-    default: {
+    default:
       d.a.packed = c.packed;
-      // Units of about 48 KiB of staged traffic.
-      const long long round_bytes = 2LL * std::max(1, c.packed.shape[0].vol) * c.elem;
-      const int rounds = (int)std::max<long long>(1, std::min<long long>(c.packed.vol_outer, (48 << 10) / round_bytes));
-      d.rounds = rounds;
-      const int groups = (c.packed.vol_outer + rounds - 1) / rounds;
+      d.items = c.packed.vol_outer;
       d.split_div = make_fastdiv((uint32_t)c.packed.num_split);
-      d.units = groups * c.packed.num_split;
+      d.item_bytes = std::max(1, 2 * std::max(1, c.packed.shape[0].vol) * c.elem / std::max(1, c.packed.num_split));
       break;
-    }
   }
+  if (d.items <= 0) return false;
+  batch_regroup(d, kBatchUnitBytes);
   return d.units > 0;
+}
+
+void batch_regroup(BatchDesc& d, long long unit_bytes) {
+  const long long r = std::max<long long>(1, std::min<long long>(d.items, unit_bytes / std::max(1, d.item_bytes)));
+  d.rounds = (int)r;
+  d.groups = (int)((d.items + r - 1) / r);
+  d.units = d.groups * (int)d.split_div.d;
+}
+
+long long batch_unit_bytes(long long total_bytes, int grid) {
+  // LIBRETT_BATCH_UNIT_KB fixes the unit size; LIBRETT_BATCH_UNITS_PER_WG
+  // sets the units each workgroup should get (default 8).
+  static const long long fixed = [] {
+    const char* e = std::getenv("LIBRETT_BATCH_UNIT_KB");
+    return e ? std::max(1LL, std::atoll(e)) << 10 : 0LL;
+  }();
+  static const long long per_wg = [] {
+    const char* e = std::getenv("LIBRETT_BATCH_UNITS_PER_WG");
+    return e ? std::max(1LL, std::atoll(e)) : 32LL;
+  }();
+  if (fixed) return fixed;
+  const long long per = total_bytes / (per_wg * std::max(1, grid));
+  return std::max<long long>(16 << 10, std::min<long long>(kBatchUnitBytes, per));
 }
 
 hipError_t launch_batch(const BatchDesc* d_descs, const int* d_start, int nprob, int total, int grid,
@@ -298,18 +326,23 @@ int batch_occupancy() {
   return n;
 }
 
-void replay_batch(const BatchDesc* descs, const int* start, int nprob, int total, int grid) {
-  std::vector<unsigned char> lds(kBatchLds + 64);
+bool replay_batch(const BatchDesc* descs, const int* start, int nprob, int total, int grid) {
+  // The workgroup's LDS, followed by a guard band that no unit may touch.
+  const size_t guard = 64 << 10;
+  std::vector<unsigned char> lds(kBatchLds + guard, 0xC3);
   for (int g = 0; g < grid; g++) {
     int u = g;
     if (u >= total) continue;
     int p = 0;
-    while (start[p + 1] <= u) p++;
+    while (p < nprob && start[p + 1] <= u) p++;
     for (; u < total; u += grid) {
       while (start[p + 1] <= u) p++;
       host_unit(descs[p], u - start[p], lds.data());
     }
   }
+  for (size_t i = kBatchLds; i < lds.size(); i++)
+    if (lds[i] != 0xC3) return false;
+  return true;
 }
 
 }  // namespace lrt

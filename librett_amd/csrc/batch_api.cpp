@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -22,6 +24,13 @@
 
 namespace lrt {
 namespace {
+
+// Internal failures name their cause on stderr when LIBRETT_DEBUG is set.
+int fail(const char* why, int code = LIBRETT_C_INTERNAL_ERROR) {
+  static const bool on = std::getenv("LIBRETT_DEBUG") != nullptr;
+  if (on) std::fprintf(stderr, "librett batch: %s\n", why);
+  return code;
+}
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -93,6 +102,7 @@ struct BatchObj {
   std::vector<int> start;               // first unit of each descriptor, then the total
   int total = 0;
   int grid = 0;
+  long long unit_bytes = kBatchUnitBytes;
   BatchDesc* d_descs = nullptr;
   int* d_start = nullptr;
   hipGraph_t graph = nullptr;
@@ -131,11 +141,13 @@ std::string problem_key(const Problem& p) {
 // Times candidate c of problem p inside the persistent kernel, on enough
 // clones of the problem to fill the device (scratch buffers `a`, `b`).
 double time_in_batch(const Candidate& c, const Problem& p, void* a, void* b, size_t scratch, hipStream_t stream,
-                     BatchDesc* d_desc, int* d_start, int max_clones, int grid) {
+                     BatchDesc* d_desc, int* d_start, int max_clones, int grid, long long unit_bytes) {
   BatchDesc d;
   if (!batch_describe(c, d)) return -1;
+  batch_regroup(d, unit_bytes);
   const size_t bytes = (size_t)p.vol * p.elem;
   const size_t span = (bytes + 255) / 256 * 256;
+  if (span > scratch) return -1;  // every clone must lie inside the scratch buffers
   int clones = (int)std::min<size_t>(max_clones, std::max<size_t>(1, scratch / span));
   clones = std::max(1, std::min(clones, (int)((64ull << 20) / std::max<size_t>(1, bytes)) + 1));
   std::vector<BatchDesc> ds(clones, d);
@@ -182,20 +194,27 @@ int build_batch(BatchObj& B, const DeviceInfo& dev, int tune) {
   std::vector<std::vector<Candidate>> ranked(B.nprob);
   for (int i = 0; i < B.nprob; i++) {
     ranked[i] = batch_candidates(B.probs[i], bdev);
-    if (ranked[i].empty()) return LIBRETT_C_INTERNAL_ERROR;
+    if (ranked[i].empty()) return fail("no batch-mode candidate");
     B.chosen[i] = ranked[i][0];
   }
   B.plan_s = now_s() - t0;
 
   const int occ = std::max(1, dev.has_gpu ? batch_occupancy() : 4);
   B.grid = dev.cu_count * occ;
+  long long total_bytes = 0;
+  for (const Problem& p : B.probs) total_bytes += 2LL * p.vol * p.elem;
+  B.unit_bytes = batch_unit_bytes(total_bytes, B.grid);
 
   if (tune && dev.has_gpu) {
     // Autotuner: time the top candidates of each distinct problem inside the
     // persistent kernel on clones of the problem, keep the fastest, and
     // remember it across batches.
     const double t1 = now_s();
-    const size_t scratch = 256ull << 20;
+    // Scratch for the clones: at least 256 MiB, and room for the largest
+    // problem up to 4 GiB (larger problems keep the model's choice).
+    size_t largest = 0;
+    for (const Problem& p : B.probs) largest = std::max(largest, ((size_t)p.vol * p.elem + 255) / 256 * 256);
+    const size_t scratch = std::max<size_t>(256ull << 20, std::min<size_t>(largest, 4ull << 30));
     void *a = nullptr, *b = nullptr;
     BatchDesc* d_desc = nullptr;
     int* d_start = nullptr;
@@ -217,7 +236,7 @@ int build_batch(BatchObj& B, const DeviceInfo& dev, int tune) {
       int bi = 0;
       for (int k = 0; k < (int)ranked[i].size() && k < topk; k++) {
         const double t = time_in_batch(ranked[i][k], B.probs[i], a, b, scratch, B.stream, d_desc, d_start,
-                                       max_clones, B.grid);
+                                       max_clones, B.grid, B.unit_bytes);
         if (t > 0 && t < best) {
           best = t;
           bi = k;
@@ -232,7 +251,7 @@ int build_batch(BatchObj& B, const DeviceInfo& dev, int tune) {
     if (b) (void)hipFree(b);
     if (d_desc) (void)hipFree(d_desc);
     if (d_start) (void)hipFree(d_start);
-    if (!ok) return LIBRETT_C_INTERNAL_ERROR;
+    if (!ok) return fail("autotuner HIP call failed");
     B.tune_s = now_s() - t1;
   }
 
@@ -244,8 +263,10 @@ int build_batch(BatchObj& B, const DeviceInfo& dev, int tune) {
   for (int i = 0; i < B.nprob; i++) {
     std::vector<BatchDesc> ds, fs;
     std::vector<long long> offs;
-    if (!describe_problem(B.probs[i], B.chosen[i], ds, fs, offs)) return LIBRETT_C_INTERNAL_ERROR;
+    if (!describe_problem(B.probs[i], B.chosen[i], ds, fs, offs)) return fail("no descriptor for the chosen plan");
     for (size_t k = 0; k < ds.size(); k++) {
+      batch_regroup(ds[k], B.unit_bytes);
+      batch_regroup(fs[k], B.unit_bytes);
       B.descs.push_back(ds[k]);
       B.fallback.push_back(fs[k]);
       B.owner.push_back(i);
@@ -389,7 +410,7 @@ int librett_c_batch_plan(unsigned int* handle, int count, const int* ranks, cons
     B->nprob = count;
     B->stream = (hipStream_t)stream;
     DeviceInfo dev;
-    if (!device_info_batch(dev)) return LIBRETT_C_INTERNAL_ERROR;
+    if (!device_info_batch(dev)) return fail("device query failed");
     B->device = dev.device_id;
     r = build_batch(*B, dev, tune);
     if (r != LIBRETT_C_SUCCESS) return r;
@@ -397,10 +418,10 @@ int librett_c_batch_plan(unsigned int* handle, int count, const int* ranks, cons
     if (hipMalloc((void**)&B->d_descs, nd * sizeof(BatchDesc)) != hipSuccess ||
         hipMalloc((void**)&B->d_start, (nd + 1) * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();
-      return LIBRETT_C_INTERNAL_ERROR;
+      return fail("descriptor allocation failed");
     }
     if (hipMemcpy(B->d_start, B->start.data(), (nd + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
-      return LIBRETT_C_INTERNAL_ERROR;
+      return fail("descriptor upload failed");
     const unsigned int h = g_next_batch.fetch_add(1);
     {
       std::lock_guard<std::mutex> lock(g_batch_mu);
@@ -408,8 +429,10 @@ int librett_c_batch_plan(unsigned int* handle, int count, const int* ranks, cons
     }
     *handle = h;
     return LIBRETT_C_SUCCESS;
+  } catch (const std::exception& e) {
+    return fail(e.what());
   } catch (...) {
-    return LIBRETT_C_INTERNAL_ERROR;
+    return fail("unknown exception");
   }
 }
 
@@ -560,13 +583,64 @@ int librett_c_batch_host_replay(int count, const int* ranks, const int* dims, co
       ds[k].in = gin[i].data() + g + B.byte_offset[k];
       ds[k].out = gout[i].data() + g + B.byte_offset[k];
     }
-    replay_batch(ds.data(), B.start.data(), (int)ds.size(), B.total, grid > 0 ? grid : B.grid);
+    if (!replay_batch(ds.data(), B.start.data(), (int)ds.size(), B.total, grid > 0 ? grid : B.grid))
+      return LIBRETT_C_UNDEFINED_ERROR;
     for (int i = 0; i < count; i++) {
       const size_t n = (size_t)B.probs[i].vol * B.probs[i].elem;
       for (size_t j = 0; j < g; j++)
         if (gout[i][j] != 0x5A || gout[i][g + n + j] != 0x5A) return LIBRETT_C_UNDEFINED_ERROR;
       std::memcpy(odata[i], gout[i].data() + g, n);
     }
+    return LIBRETT_C_SUCCESS;
+  } catch (...) {
+    return LIBRETT_C_INTERNAL_ERROR;
+  }
+}
+
+int librett_c_batch_host_candidate(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                                   const librett_c_device* d, int index, int fallback, long long unit_bytes,
+                                   int grid, const void* idata, void* odata, int* count) {
+  try {
+    std::vector<Problem> probs;
+    const size_t elems[1] = {sizeofType};
+    int r = read_problems(1, &rank, dim, permutation, elems, probs);
+    if (r != LIBRETT_C_SUCCESS) return r;
+    DeviceInfo dev;
+    if (d) {
+      dev.cu_count = d->cu_count;
+      dev.lds_per_block = d->lds_per_block;
+      dev.lds_per_cu = d->lds_per_cu;
+      dev.max_threads_per_block = d->max_threads_per_block;
+      dev.max_waves_per_cu = d->max_waves_per_cu;
+      dev.hbm_bytes_per_s = d->hbm_bytes_per_s;
+    }
+    dev.has_gpu = 0;
+    const Problem& p = probs[0];
+    const std::vector<Candidate> ranked = batch_candidates(p, dev);
+    if (count) *count = (int)ranked.size();
+    if (!idata && !odata) return LIBRETT_C_SUCCESS;
+    if (!idata || !odata || index < 0 || index >= (int)ranked.size()) return LIBRETT_C_INVALID_PARAMETER;
+    std::vector<BatchDesc> ds, fs;
+    std::vector<long long> offs;
+    if (!describe_problem(p, ranked[index], ds, fs, offs)) return LIBRETT_C_INTERNAL_ERROR;
+    if (fallback) ds = fs;
+    const long long ub = unit_bytes > 0 ? unit_bytes : kBatchUnitBytes;
+    const size_t n = (size_t)p.vol * p.elem;
+    const size_t g = 4096;
+    std::vector<unsigned char> gin(n + 2 * g, 0xA5), gout(n + 2 * g, 0x5A);
+    std::memcpy(gin.data() + g, idata, n);
+    std::vector<int> start(ds.size() + 1, 0);
+    for (size_t k = 0; k < ds.size(); k++) {
+      batch_regroup(ds[k], ub);
+      ds[k].in = gin.data() + g + offs[k];
+      ds[k].out = gout.data() + g + offs[k];
+      start[k + 1] = start[k] + ds[k].units;
+    }
+    const int gr = grid > 0 ? grid : dev.cu_count * 4;
+    if (!replay_batch(ds.data(), start.data(), (int)ds.size(), start.back(), gr)) return LIBRETT_C_UNDEFINED_ERROR;
+    for (size_t j = 0; j < g; j++)
+      if (gout[j] != 0x5A || gout[g + n + j] != 0x5A) return LIBRETT_C_UNDEFINED_ERROR;
+    std::memcpy(odata, gout.data() + g, n);
     return LIBRETT_C_SUCCESS;
   } catch (...) {
     return LIBRETT_C_INTERNAL_ERROR;

@@ -361,6 +361,10 @@ class Server {
       take.push_back(r);
     }
     if (take.empty()) return false;
+    long long bytes = 0;
+    for (auto& r : take) bytes += 2LL * r->prob.vol * r->prob.elem;
+    const long long unit_bytes = batch_unit_bytes(bytes, grid_);
+    for (int k = 0; k < ndesc; k++) batch_regroup(use[k], unit_bytes);
     int total = 0;
     for (int k = 0; k < ndesc; k++) {
       s.h_desc[k] = use[k];

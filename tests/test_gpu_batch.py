@@ -133,3 +133,22 @@ def test_batch_autotuned(gpu_stream):
     assert not slab.check()
     b.destroy()
     slab.free()
+
+
+def test_batch_autotuned_problem_over_256mib(gpu_stream):
+    # The autotuner times clones of each problem in scratch buffers sized for
+    # the largest problem of the batch; a 269.5 MB problem must fit them.
+    probs = [([5, 97, 211, 331], [3, 1, 0, 2], 8), ([64, 48, 32], [2, 0, 1], 8), ([33, 7, 130], [2, 0, 1], 16)]
+    say("case: autotuned batch with a 269.5 MB problem")
+    slab = Slab(probs, seed=7)
+    b = L.Batch(probs, gpu_stream, tune=2)
+    info = b.info()
+    say("  info", info)
+    assert info["tuned"] >= 1
+    b.set_buffers(slab.in_ptrs(), slab.out_ptrs())
+    hip.memset(slab.b.ptr, 0xEE, slab.total)
+    assert b.execute(use_graph=True) == L.LIBRETT_SUCCESS
+    hip.stream_sync(gpu_stream)
+    assert not slab.check()
+    b.destroy()
+    slab.free()
