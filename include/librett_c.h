@@ -242,6 +242,74 @@ int librett_c_server_run_poisson(unsigned int handle, const librett_c_poisson* c
 /* Extension: library version string. */
 const char* librett_c_version(void);
 
+/* ------------------------------------------------------------------------
+ * Learned plan selector (extension; the reference ranks candidates with its
+ * analytic model only, src/plan.cpp:721-789). A 2 x 64 FP32 MLP scores
+ * every candidate plan of a problem from LIBRETT_C_SELECTOR_FEATURES
+ * features; librettPlan / librett_c_plan take its lowest score when
+ * weights are loaded (librett_c_selector_load / _set, or the file named by
+ * LIBRETT_SELECTOR_WEIGHTS) and the analytic model otherwise, or when the
+ * reduced rank exceeds 7.
+ * ------------------------------------------------------------------------ */
+#define LIBRETT_C_SELECTOR_FEATURES 48
+
+/* Input features, hidden width and parameter count of the MLP. */
+void librett_c_mlp_sizes(int* nfeat, int* hidden, int* nparams);
+
+/* Times every candidate of a problem with librettPlanMeasure's own loop
+ * (warm-up, then the median of LIBRETT_MEASURE_REPS runs, each after an
+ * output memset); times[i] in seconds, negative if candidate i did not run.
+ * Candidate order is librett_c_candidates'. The selector's training labels. */
+int librett_c_measure_candidates(int rank, const int* dim, const int* permutation, size_t sizeofType, void* stream,
+                                 const void* idata, void* odata, double* times, int max, int* count);
+
+/* Raw features (count x LIBRETT_C_SELECTOR_FEATURES floats, candidate order
+ * as above) and the analytic model's choice. `device` NULL: the current GPU;
+ * otherwise host-only planning for the described device. */
+int librett_c_selector_features(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                                const librett_c_device* device, float* feats, int max, int* count,
+                                int* model_choice);
+
+/* Candidate the loaded selector picks on the current GPU (-1: analytic). */
+int librett_c_selector_choose(int rank, const int* dim, const int* permutation, size_t sizeofType, int* index);
+
+/* Loads weights: nparams floats in the layout W1[64][48], b1[64], W2[64][64],
+ * b2[64], w3[64], b3, with per-feature mean and std used to normalise the
+ * inputs. params NULL unloads. */
+int librett_c_selector_set(const float* params, int nparams, const float* mean, const float* std, int nfeat);
+
+/* Loads a weights file: uint32 magic 0x4D54524C, version 1, 48, 64, then
+ * float32 mean[48], std[48], params[nparams] (little-endian). */
+int librett_c_selector_load(const char* path);
+
+/* Generation of the loaded weights (0: none). */
+unsigned long long librett_c_selector_generation(void);
+
+/* MLP kernels on device pointers. forward: out[n] = MLP(x[n][48]).
+ * grad: workgroup b (of nblocks) writes the gradient of samples
+ * idx[base + 128 b .. min(end, base + 128 (b+1))) of sum (out - y)^2 * inv_m
+ * to partial[b][nparams] and its squared-error sum to loss[b].
+ * adam: params -= lr * mhat / (sqrt(vhat) + eps) with the gradient summed
+ * over the nparts partials; c1 = 1/(1-beta1^t), c2 = 1/(1-beta2^t).
+ * train: Adam over `epochs` shuffled passes (std::mt19937(seed)) with
+ * minibatches of `minibatch` samples (a multiple of 128); epoch_loss (host,
+ * may be NULL) gets the mean squared error of each epoch. */
+int librett_c_mlp_forward(const float* params, const float* x, int n, float* out, void* stream);
+int librett_c_mlp_grad(const float* params, const float* x, const float* y, const int* idx, int base, int end,
+                       float inv_m, int nblocks, float* partial, float* loss, void* stream);
+int librett_c_mlp_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr, float beta1,
+                       float beta2, float c1, float c2, float eps, void* stream);
+int librett_c_mlp_train(const float* x, const float* y, int n, float* params, int epochs, float lr, int minibatch,
+                        unsigned int seed, float* epoch_loss, void* stream);
+
+/* Host replays of the same kernels' per-thread code (one workgroup for
+ * grad: count <= 128 samples idx[0..count)). */
+void librett_c_mlp_host_forward(const float* params, const float* x, int n, float* out);
+int librett_c_mlp_host_grad(const float* params, const float* x, const float* y, const int* idx, int count,
+                            float inv_m, float* grad, float* loss);
+void librett_c_mlp_host_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr,
+                             float beta1, float beta2, float c1, float c2, float eps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,10 @@ EXPORTED = [
     "librett_c_server_create", "librett_c_server_alloc", "librett_c_server_free", "librett_c_server_submit",
     "librett_c_server_wait", "librett_c_server_drain", "librett_c_server_stats_get", "librett_c_server_reset_stats",
     "librett_c_server_destroy", "librett_c_server_run_poisson",
+    "librett_c_mlp_sizes", "librett_c_measure_candidates", "librett_c_selector_features", "librett_c_selector_choose",
+    "librett_c_selector_set", "librett_c_selector_load", "librett_c_selector_generation", "librett_c_mlp_forward",
+    "librett_c_mlp_grad", "librett_c_mlp_adam", "librett_c_mlp_train", "librett_c_mlp_host_forward",
+    "librett_c_mlp_host_grad", "librett_c_mlp_host_adam",
 ]
 LIBRETT_BUSY = 6
 
@@ -167,12 +171,35 @@ def lib():
         L.librett_c_server_reset_stats.argtypes = [ctypes.c_uint]
         L.librett_c_server_destroy.argtypes = [ctypes.c_uint]
         L.librett_c_server_run_poisson.argtypes = [ctypes.c_uint, ctypes.POINTER(PoissonConfig), vp, ctypes.c_size_t]
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        f_ = ctypes.c_float
+        L.librett_c_mlp_sizes.argtypes = [ip, ip, ip]
+        L.librett_c_measure_candidates.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, vp, vp, vp, dp, ctypes.c_int,
+                                                   ip]
+        L.librett_c_selector_features.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, ctypes.POINTER(DeviceDesc),
+                                                  fp, ctypes.c_int, ip, ip]
+        L.librett_c_selector_choose.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, ip]
+        L.librett_c_selector_set.argtypes = [fp, ctypes.c_int, fp, fp, ctypes.c_int]
+        L.librett_c_selector_load.argtypes = [ctypes.c_char_p]
+        L.librett_c_selector_generation.argtypes = []
+        L.librett_c_mlp_forward.argtypes = [vp, vp, ctypes.c_int, vp, vp]
+        L.librett_c_mlp_grad.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, f_, ctypes.c_int, vp, vp, vp]
+        L.librett_c_mlp_adam.argtypes = [vp, ctypes.c_int, vp, vp, vp, f_, f_, f_, f_, f_, f_, vp]
+        L.librett_c_mlp_train.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, f_, ctypes.c_int, ctypes.c_uint, fp,
+                                          vp]
+        L.librett_c_mlp_host_forward.argtypes = [fp, fp, ctypes.c_int, fp]
+        L.librett_c_mlp_host_grad.argtypes = [fp, fp, fp, ip, ctypes.c_int, f_, fp, fp]
+        L.librett_c_mlp_host_adam.argtypes = [fp, ctypes.c_int, fp, fp, fp, f_, f_, f_, f_, f_, f_]
         L.librett_c_version.restype = ctypes.c_char_p
         for name in EXPORTED:
             f = getattr(L, name)
             if name != "librett_c_version" and f.restype is not ctypes.c_char_p:
                 f.restype = ctypes.c_int
         L.librett_c_set_model_constants.restype = None
+        for name in ("librett_c_mlp_sizes", "librett_c_mlp_host_forward", "librett_c_mlp_host_adam"):
+            getattr(L, name).restype = None
+        L.librett_c_selector_generation.restype = ctypes.c_ulonglong
         L.librett_c_initialize.restype = None
         L.librett_c_finalize.restype = None
         _LIB = L
@@ -445,6 +472,138 @@ def batch_host_replay_candidate(dim, permutation, sizeofType, x, index, fallback
                                              ctypes.byref(device or mi355x_desc()), int(index), int(bool(fallback)),
                                              int(unit_bytes), int(grid), x.ctypes.data, y.ctypes.data, ctypes.byref(n))
     return r, y
+
+
+# ---------------------------------------------------------------------------
+# Learned plan selector (extension)
+# ---------------------------------------------------------------------------
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def mlp_sizes():
+    """(features, hidden width, parameter count) of the selector's MLP."""
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().librett_c_mlp_sizes(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value
+
+
+def measure_candidates(dim, permutation, sizeofType, stream, idata, odata, max_count=1024):
+    """Times of every candidate (librettPlanMeasure's loop), in seconds."""
+    import numpy as np
+    t = np.full(max_count, -1.0)
+    n = ctypes.c_int(0)
+    r = lib().librett_c_measure_candidates(len(dim), _iarr(dim), _iarr(permutation), int(sizeofType),
+                                           ctypes.c_void_p(stream), ctypes.c_void_p(idata), ctypes.c_void_p(odata),
+                                           t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), max_count,
+                                           ctypes.byref(n))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("measure_candidates failed: %d" % r)
+    return t[:min(n.value, max_count)].copy()
+
+
+def selector_features(dim, permutation, sizeofType, device=None, max_count=1024):
+    """(features [n, 48] float32, analytic model's choice). device None: the current GPU."""
+    import numpy as np
+    nf = mlp_sizes()[0]
+    f = np.zeros((max_count, nf), dtype=np.float32)
+    n, mc = ctypes.c_int(0), ctypes.c_int(-1)
+    r = lib().librett_c_selector_features(len(dim), _iarr(dim), _iarr(permutation), int(sizeofType),
+                                          ctypes.byref(device) if device is not None else None, _fptr(f), max_count,
+                                          ctypes.byref(n), ctypes.byref(mc))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("selector_features failed: %d" % r)
+    return f[:min(n.value, max_count)].copy(), mc.value
+
+
+def selector_choose(dim, permutation, sizeofType):
+    i = ctypes.c_int(-1)
+    r = lib().librett_c_selector_choose(len(dim), _iarr(dim), _iarr(permutation), int(sizeofType), ctypes.byref(i))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("selector_choose failed: %d" % r)
+    return i.value
+
+
+def selector_set(params, mean, std):
+    import numpy as np
+    if params is None:
+        return lib().librett_c_selector_set(None, 0, None, None, 0)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    m = np.ascontiguousarray(mean, dtype=np.float32)
+    s = np.ascontiguousarray(std, dtype=np.float32)
+    return lib().librett_c_selector_set(_fptr(p), p.size, _fptr(m), _fptr(s), m.size)
+
+
+def selector_load(path):
+    return lib().librett_c_selector_load(os.fsencode(path))
+
+
+def selector_generation():
+    return lib().librett_c_selector_generation()
+
+
+def mlp_host_forward(params, x):
+    import numpy as np
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros(xx.shape[0], dtype=np.float32)
+    lib().librett_c_mlp_host_forward(_fptr(p), _fptr(xx), xx.shape[0], _fptr(out))
+    return out
+
+
+def mlp_host_grad(params, x, y, idx, inv_m):
+    """Host replay of one training workgroup: (gradient, squared-error sum)."""
+    import numpy as np
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    yy = np.ascontiguousarray(y, dtype=np.float32)
+    ii = np.ascontiguousarray(idx, dtype=np.int32)
+    g = np.zeros(p.size, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    r = lib().librett_c_mlp_host_grad(_fptr(p), _fptr(xx), _fptr(yy), ii.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      ii.size, float(inv_m), _fptr(g), _fptr(loss))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("mlp_host_grad failed: %d" % r)
+    return g, float(loss[0])
+
+
+def mlp_host_adam(partials, params, m, v, lr, beta1, beta2, c1, c2, eps):
+    """Host replay of the Adam kernel; updates params, m, v (float32 arrays) in place."""
+    import numpy as np
+    g = np.ascontiguousarray(partials, dtype=np.float32)
+    for a in (params, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    lib().librett_c_mlp_host_adam(_fptr(g), g.shape[0], _fptr(params), _fptr(m), _fptr(v), lr, beta1, beta2, c1, c2,
+                                  eps)
+
+
+def _ok(r, what):
+    if r != LIBRETT_SUCCESS:
+        raise RuntimeError("%s failed: %d" % (what, r))
+
+
+def mlp_forward(d_params, d_x, n, d_out, stream):
+    _ok(lib().librett_c_mlp_forward(d_params, d_x, n, d_out, ctypes.c_void_p(stream)), "mlp_forward")
+
+
+def mlp_grad(d_params, d_x, d_y, d_idx, base, end, inv_m, nblocks, d_partial, d_loss, stream):
+    _ok(lib().librett_c_mlp_grad(d_params, d_x, d_y, d_idx, base, end, inv_m, nblocks, d_partial, d_loss,
+                                 ctypes.c_void_p(stream)), "mlp_grad")
+
+
+def mlp_adam(d_partial, nparts, d_params, d_m, d_v, lr, beta1, beta2, c1, c2, eps, stream):
+    _ok(lib().librett_c_mlp_adam(d_partial, nparts, d_params, d_m, d_v, lr, beta1, beta2, c1, c2, eps,
+                                 ctypes.c_void_p(stream)), "mlp_adam")
+
+
+def mlp_train(d_x, d_y, n, d_params, epochs, lr, minibatch, seed, stream):
+    """Trains on the GPU; returns the per-epoch mean squared error."""
+    import numpy as np
+    loss = np.zeros(max(1, epochs), dtype=np.float32)
+    _ok(lib().librett_c_mlp_train(d_x, d_y, n, d_params, epochs, lr, minibatch, seed, _fptr(loss),
+                                  ctypes.c_void_p(stream)), "mlp_train")
+    return loss[:epochs]
 
 
 def version():

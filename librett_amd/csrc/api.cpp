@@ -30,6 +30,7 @@
 #include "kernels.h"
 #include "lru_cache.h"
 #include "planner.h"
+#include "selector.h"
 
 namespace lrt {
 namespace {
@@ -132,8 +133,9 @@ DeviceInfo host_device(const librett_c_device* d) {
 
 std::string cache_key(const Problem& p, int device, bool measured) {
   std::string k;
-  k.reserve(16 + p.rank * 8);
+  k.reserve(24 + p.rank * 8);
   k += measured ? 'M' : 'H';
+  if (!measured) k += std::to_string(selector_generation()) + '.';  // heuristic choices depend on the selector
   k += std::to_string(device);
   k += ':';
   k += std::to_string(p.elem);
@@ -296,6 +298,32 @@ void fill_info(const Candidate& c, double measured, librett_c_plan_info* info) {
   std::snprintf(info->desc, sizeof(info->desc), "%s", c.describe().c_str());
 }
 
+// librettPlanMeasure's timing of every candidate (src/librett.cpp:224-343
+// times each candidate once; here each gets a warm-up and the median of
+// LIBRETT_MEASURE_REPS runs). Times what librettExecute would run on these
+// pointers; a Trivial candidate is exclusive and ends the list. Negative
+// entries are candidates that failed to run.
+std::vector<double> measure_all(const Problem& full, const std::vector<Candidate>& cands, const DeviceInfo& dev,
+                                hipStream_t stream, const void* in, void* out) {
+  const int reps = std::max(1, env_int("LIBRETT_MEASURE_REPS", 3));
+  std::vector<double> ts(cands.size(), -1.0);
+  for (int i = 0; i < (int)cands.size(); i++) {
+    auto probe = make_plan_obj(full, cands[i], dev.device_id, stream);
+    const uintptr_t mis = ((uintptr_t)in | (uintptr_t)out) & (uintptr_t)(probe->align - 1);
+    const Candidate& run = mis ? probe->fallback : probe->cand;
+    ts[i] = time_candidate(run, stream, in, out, cands[i].method == kTrivial ? 1 : reps);
+    if (cands[i].method == kTrivial) break;
+  }
+  return ts;
+}
+
+// Features of every candidate and the analytic model's choice.
+void candidate_features(const Problem& full, const std::vector<Candidate>& cands, float* feats, int max) {
+  const Problem red = reduce_ranks(full);
+  for (int i = 0; i < (int)cands.size() && i < max; i++)
+    selector_features(full, red, cands[i], feats + (size_t)i * kSelFeatures);
+}
+
 // Shared body of librettPlan / librettPlanMeasure / the forced-candidate plan.
 // mode 0: model, 1: measure, 2: candidate `index`.
 librettResult plan_impl(unsigned int* handle, int rank, const int* dim, const int* perm, size_t sizeofType,
@@ -320,25 +348,21 @@ librettResult plan_impl(unsigned int* handle, int rank, const int* dim, const in
       if (cands.empty()) return LIBRETT_INTERNAL_ERROR;
       int best = -1;
       if (mode == 0) {
-        best = choose_by_model(cands);
+        // The learned selector when weights are loaded, else (or when it
+        // declines the problem) the analytic model.
+        best = selector_choose(cands, full, dev.device_id);
+        if (best < 0) best = choose_by_model(cands);
       } else if (mode == 2) {
         if (index < 0 || index >= (int)cands.size()) return LIBRETT_INVALID_PARAMETER;
         best = index;
       } else {
-        const int reps = std::max(1, env_int("LIBRETT_MEASURE_REPS", 3));
+        const std::vector<double> ts = measure_all(full, cands, dev, stream, in, out);
         double bt = 1e30;
-        for (int i = 0; i < (int)cands.size(); i++) {
-          // Time what librettExecute would run on these pointers.
-          auto probe = make_plan_obj(full, cands[i], dev.device_id, stream);
-          const uintptr_t mis = ((uintptr_t)in | (uintptr_t)out) & (uintptr_t)(probe->align - 1);
-          const Candidate& run = mis ? probe->fallback : probe->cand;
-          const double t = time_candidate(run, stream, in, out, cands[i].method == kTrivial ? 1 : reps);
-          if (t >= 0 && t < bt) {
-            bt = t;
+        for (int i = 0; i < (int)ts.size(); i++)
+          if (ts[i] >= 0 && ts[i] < bt) {
+            bt = ts[i];
             best = i;
           }
-          if (cands[i].method == kTrivial) break;
-        }
         measured = bt;
       }
       if (best < 0) return LIBRETT_INTERNAL_ERROR;
@@ -486,6 +510,62 @@ int librett_c_host_replay(int rank, const int* dim, const int* permutation, size
     for (size_t i = 0; i < g; i++)
       if (out[i] != 0x5A || out[g + n + i] != 0x5A) return LIBRETT_UNDEFINED_ERROR;
     std::memcpy(odata, out.data() + g, n);
+    return LIBRETT_SUCCESS;
+  } catch (...) {
+    return LIBRETT_INTERNAL_ERROR;
+  }
+}
+
+int librett_c_measure_candidates(int rank, const int* dim, const int* permutation, size_t sizeofType, void* stream,
+                                 const void* idata, void* odata, double* times, int max, int* count) {
+  librettResult r = check_input(rank, dim, permutation, sizeofType);
+  if (r != LIBRETT_SUCCESS) return r;
+  if (!idata || !odata || idata == odata) return LIBRETT_INVALID_PARAMETER;
+  try {
+    DeviceInfo dev;
+    if (!device_info(dev)) return LIBRETT_INTERNAL_ERROR;
+    const Problem full = canonicalize(rank, dim, permutation, (int)sizeofType);
+    const std::vector<Candidate> cands = make_candidates(full, dev);
+    const std::vector<double> ts = measure_all(full, cands, dev, (hipStream_t)stream, idata, odata);
+    if (count) *count = (int)cands.size();
+    for (int i = 0; i < (int)ts.size() && i < max && times; i++) times[i] = ts[i];
+    return LIBRETT_SUCCESS;
+  } catch (...) {
+    return LIBRETT_INTERNAL_ERROR;
+  }
+}
+
+int librett_c_selector_features(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                                const librett_c_device* d, float* feats, int max, int* count, int* model_choice) {
+  librettResult r = check_input(rank, dim, permutation, sizeofType);
+  if (r != LIBRETT_SUCCESS) return r;
+  try {
+    DeviceInfo dev;
+    if (d)
+      dev = host_device(d);
+    else if (!device_info(dev))
+      return LIBRETT_INTERNAL_ERROR;
+    const Problem full = canonicalize(rank, dim, permutation, (int)sizeofType);
+    const std::vector<Candidate> cands = make_candidates(full, dev);
+    if (count) *count = (int)cands.size();
+    if (model_choice) *model_choice = choose_by_model(cands);
+    if (feats) candidate_features(full, cands, feats, max);
+    return LIBRETT_SUCCESS;
+  } catch (...) {
+    return LIBRETT_INTERNAL_ERROR;
+  }
+}
+
+int librett_c_selector_choose(int rank, const int* dim, const int* permutation, size_t sizeofType, int* index) {
+  librettResult r = check_input(rank, dim, permutation, sizeofType);
+  if (r != LIBRETT_SUCCESS) return r;
+  if (!index) return LIBRETT_INVALID_PARAMETER;
+  try {
+    DeviceInfo dev;
+    if (!device_info(dev)) return LIBRETT_INTERNAL_ERROR;
+    const Problem full = canonicalize(rank, dim, permutation, (int)sizeofType);
+    const std::vector<Candidate> cands = make_candidates(full, dev);
+    *index = selector_choose(cands, full, dev.device_id);
     return LIBRETT_SUCCESS;
   } catch (...) {
     return LIBRETT_INTERNAL_ERROR;

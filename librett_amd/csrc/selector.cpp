@@ -1,0 +1,338 @@
+// Learned plan selector: features, loaded weights, GPU inference for
+// librettPlan, and the C-ABI of the MLP kernels (training and inference).
+//
+// The reference chooses among candidates with its analytic model only
+// (choosePlanHeuristic, src/plan.cpp:721-789); its librettPlanMeasure
+// (src/librett.cpp:224-343) times every candidate, which is where the
+// selector's training labels come from.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <random>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/librett_c.h"
+#include "selector.h"
+#include "selector_ops.h"
+
+namespace lrt {
+
+static_assert(mlp::F == kSelFeatures, "feature count");
+
+namespace {
+
+float lg(double v) { return v > 0 ? (float)std::log2(v) : 0.0f; }
+
+struct DevBuf {
+  float* params = nullptr;
+  float* x = nullptr;
+  float* out = nullptr;
+  int cap = 0;
+  uint64_t gen = 0;
+  hipStream_t stream = nullptr;
+};
+
+// Weights and per-device inference buffers. The buffers live until the
+// process ends (planning may run on any thread at any time).
+struct Selector {
+  std::mutex mu;
+  uint64_t gen = 0;    // 0: no weights
+  uint64_t next = 1;
+  std::vector<float> params, mean, inv_std;
+  std::unordered_map<int, DevBuf> dev;
+};
+
+Selector& state() {
+  static Selector* s = new Selector;
+  return *s;
+}
+
+constexpr uint32_t kMagic = 0x4D54524C;  // "LRTM"
+
+int set_weights(const float* params, int nparams, const float* mean, const float* stdv, int nfeat) {
+  Selector& s = state();
+  std::lock_guard<std::mutex> lock(s.mu);
+  if (!params) {
+    s.gen = 0;
+    return LIBRETT_C_SUCCESS;
+  }
+  if (nparams != mlp::P || nfeat != mlp::F || !mean || !stdv) return LIBRETT_C_INVALID_PARAMETER;
+  for (int i = 0; i < nparams; i++)
+    if (!std::isfinite(params[i])) return LIBRETT_C_INVALID_PARAMETER;
+  s.params.assign(params, params + nparams);
+  s.mean.assign(mean, mean + nfeat);
+  s.inv_std.resize(nfeat);
+  for (int i = 0; i < nfeat; i++) s.inv_std[i] = stdv[i] > 1e-6f ? 1.0f / stdv[i] : 1.0f;
+  s.gen = s.next++;
+  return LIBRETT_C_SUCCESS;
+}
+
+int load_weights(const char* path) {
+  if (!path) return LIBRETT_C_INVALID_PARAMETER;
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) return LIBRETT_C_INVALID_PARAMETER;
+  uint32_t hdr[4] = {0, 0, 0, 0};
+  std::vector<float> mean(mlp::F), stdv(mlp::F), params(mlp::P);
+  bool ok = std::fread(hdr, sizeof(uint32_t), 4, fp) == 4 && hdr[0] == kMagic && hdr[1] == 1 &&
+            (int)hdr[2] == mlp::F && (int)hdr[3] == mlp::H;
+  ok = ok && std::fread(mean.data(), sizeof(float), mlp::F, fp) == (size_t)mlp::F;
+  ok = ok && std::fread(stdv.data(), sizeof(float), mlp::F, fp) == (size_t)mlp::F;
+  ok = ok && std::fread(params.data(), sizeof(float), mlp::P, fp) == (size_t)mlp::P;
+  std::fclose(fp);
+  if (!ok) return LIBRETT_C_INVALID_PARAMETER;
+  return set_weights(params.data(), mlp::P, mean.data(), stdv.data(), mlp::F);
+}
+
+// Weights named by LIBRETT_SELECTOR_WEIGHTS are loaded on first use.
+void load_from_env() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const char* p = std::getenv("LIBRETT_SELECTOR_WEIGHTS");
+    if (p && *p && load_weights(p) != LIBRETT_C_SUCCESS)
+      std::fprintf(stderr, "librett: cannot load selector weights from %s; using the analytic model\n", p);
+  });
+}
+
+}  // namespace
+
+void selector_features(const Problem& full, const Problem& red, const Candidate& c, float* f) {
+  for (int i = 0; i < mlp::F; i++) f[i] = 0.0f;
+  f[0] = (float)red.rank;
+  for (int i = 0; i < kSelMaxRank && i < red.rank; i++) {
+    f[1 + i] = lg(red.dim[i]);
+    f[8 + i] = (float)(red.perm[i] + 1);
+  }
+  f[15] = lg(full.elem);
+  f[16] = lg((double)full.vol * full.elem);
+  f[17] = (float)full.rank;
+  if (c.method >= kTrivial && c.method <= kTiledCopy) f[18 + (c.method - kTrivial)] = 1.0f;
+  f[23] = lg(c.tile_x);
+  f[24] = lg(c.tile_y);
+  f[25] = lg(c.vec_x);
+  f[26] = lg(c.vec_y);
+  f[27] = lg(c.word);
+  f[28] = lg(c.copy_tx);
+  f[29] = (float)c.nreg;
+  f[30] = lg(c.split.numSplit);
+  f[31] = lg(c.split.volMmk);
+  f[32] = lg(c.split.volMbar);
+  f[33] = lg(c.split.volMm);
+  f[34] = lg(c.split.volMk);
+  f[35] = lg((double)c.grid_x * c.grid_y);
+  f[36] = lg(c.block);
+  f[37] = c.lds_bytes / 1024.0f;
+  f[38] = (float)c.active_blocks_per_cu;
+  const ModelStats& m = c.model;
+  f[39] = lg(1 + m.read_lines);
+  f[40] = lg(1 + m.write_lines);
+  f[41] = lg(1 + m.partial_lines);
+  f[42] = lg(1 + m.read_tran);
+  f[43] = lg(1 + m.write_tran);
+  f[44] = lg(1 + m.items);
+  f[45] = lg(1 + m.steps_per_block);
+  // Modelled times relative to the HBM roofline of the move (8 TB/s).
+  const double t_roof = std::max(1.0, (double)c.bytes) / 8.0e12;
+  f[46] = lg(m.predicted_s / t_roof);
+  f[47] = lg(m.t_stream / t_roof);
+}
+
+uint64_t selector_generation() {
+  load_from_env();
+  Selector& s = state();
+  std::lock_guard<std::mutex> lock(s.mu);
+  return s.gen;
+}
+
+int selector_choose(const std::vector<Candidate>& cands, const Problem& full, int device) {
+  load_from_env();
+  Selector& s = state();
+  std::lock_guard<std::mutex> lock(s.mu);
+  if (!s.gen || cands.empty()) return -1;
+  for (int i = 0; i < (int)cands.size(); i++)
+    if (cands[i].method == kTrivial) return i;  // a plain copy is exclusive, as in the analytic choice
+  const Problem red = reduce_ranks(full);
+  if (red.rank > kSelMaxRank) return -1;
+  const int n = (int)cands.size();
+  std::vector<float> x((size_t)n * mlp::F);
+  for (int i = 0; i < n; i++) {
+    float* f = x.data() + (size_t)i * mlp::F;
+    selector_features(full, red, cands[i], f);
+    for (int k = 0; k < mlp::F; k++) {
+      f[k] = (f[k] - s.mean[k]) * s.inv_std[k];
+      if (!std::isfinite(f[k])) return -1;
+    }
+  }
+  DevBuf& d = s.dev[device];
+  bool ok = true;
+  if (!d.stream) ok = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) == hipSuccess;
+  if (ok && !d.params) ok = hipMalloc((void**)&d.params, mlp::P * sizeof(float)) == hipSuccess;
+  if (ok && d.cap < n) {
+    if (d.x) (void)hipFree(d.x);
+    if (d.out) (void)hipFree(d.out);
+    d.x = d.out = nullptr;
+    d.cap = 0;
+    const int cap = (n + 63) / 64 * 64;
+    ok = hipMalloc((void**)&d.x, (size_t)cap * mlp::F * sizeof(float)) == hipSuccess &&
+         hipMalloc((void**)&d.out, (size_t)cap * sizeof(float)) == hipSuccess;
+    if (ok) d.cap = cap;
+  }
+  if (ok && d.gen != s.gen) {
+    ok = hipMemcpyAsync(d.params, s.params.data(), mlp::P * sizeof(float), hipMemcpyHostToDevice, d.stream) ==
+         hipSuccess;
+    if (ok) d.gen = s.gen;
+  }
+  std::vector<float> score(n);
+  ok = ok &&
+       hipMemcpyAsync(d.x, x.data(), x.size() * sizeof(float), hipMemcpyHostToDevice, d.stream) == hipSuccess &&
+       mlp::launch_forward(d.params, d.x, n, d.out, d.stream) == hipSuccess &&
+       hipMemcpyAsync(score.data(), d.out, n * sizeof(float), hipMemcpyDeviceToHost, d.stream) == hipSuccess &&
+       hipStreamSynchronize(d.stream) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    d.gen = 0;
+    return -1;
+  }
+  int best = -1;
+  for (int i = 0; i < n; i++) {
+    if (!std::isfinite(score[i])) return -1;
+    if (best < 0 || score[i] < score[best]) best = i;
+  }
+  return best;
+}
+
+}  // namespace lrt
+
+using namespace lrt;
+
+extern "C" {
+
+void librett_c_mlp_sizes(int* nfeat, int* hidden, int* nparams) {
+  if (nfeat) *nfeat = mlp::F;
+  if (hidden) *hidden = mlp::H;
+  if (nparams) *nparams = mlp::P;
+}
+
+int librett_c_selector_set(const float* params, int nparams, const float* mean, const float* stdv, int nfeat) {
+  return set_weights(params, nparams, mean, stdv, nfeat);
+}
+
+int librett_c_selector_load(const char* path) { return load_weights(path); }
+
+unsigned long long librett_c_selector_generation(void) { return selector_generation(); }
+
+int librett_c_mlp_forward(const float* params, const float* x, int n, float* out, void* stream) {
+  if (!params || !x || !out || n < 0) return LIBRETT_C_INVALID_PARAMETER;
+  if (mlp::launch_forward(params, x, n, out, (hipStream_t)stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return LIBRETT_C_INTERNAL_ERROR;
+  }
+  return LIBRETT_C_SUCCESS;
+}
+
+int librett_c_mlp_grad(const float* params, const float* x, const float* y, const int* idx, int base, int end,
+                       float inv_m, int nblocks, float* partial, float* loss, void* stream) {
+  if (!params || !x || !y || !idx || !partial || !loss || nblocks < 1 || base < 0 || end < base)
+    return LIBRETT_C_INVALID_PARAMETER;
+  if (mlp::prepare_grad() != hipSuccess ||
+      mlp::launch_grad(params, x, y, idx, base, end, inv_m, nblocks, partial, loss, (hipStream_t)stream) !=
+          hipSuccess) {
+    (void)hipGetLastError();
+    return LIBRETT_C_INTERNAL_ERROR;
+  }
+  return LIBRETT_C_SUCCESS;
+}
+
+int librett_c_mlp_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr, float beta1,
+                       float beta2, float c1, float c2, float eps, void* stream) {
+  if (!partial || !params || !m || !v || nparts < 1) return LIBRETT_C_INVALID_PARAMETER;
+  if (mlp::launch_adam(partial, nparts, params, m, v, lr, beta1, beta2, c1, c2, eps, (hipStream_t)stream) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    return LIBRETT_C_INTERNAL_ERROR;
+  }
+  return LIBRETT_C_SUCCESS;
+}
+
+int librett_c_mlp_train(const float* x, const float* y, int n, float* params, int epochs, float lr, int minibatch,
+                        unsigned int seed, float* epoch_loss, void* stream_) {
+  if (!x || !y || !params || n < 1 || epochs < 0 || minibatch < mlp::S) return LIBRETT_C_INVALID_PARAMETER;
+  hipStream_t stream = (hipStream_t)stream_;
+  const int nblocks = minibatch / mlp::S;
+  const int mb = nblocks * mlp::S;
+  int* d_idx = nullptr;
+  float *d_part = nullptr, *d_m = nullptr, *d_v = nullptr, *d_loss = nullptr;
+  bool ok = hipMalloc((void**)&d_idx, (size_t)n * sizeof(int)) == hipSuccess &&
+            hipMalloc((void**)&d_part, (size_t)nblocks * mlp::P * sizeof(float)) == hipSuccess &&
+            hipMalloc((void**)&d_m, mlp::P * sizeof(float)) == hipSuccess &&
+            hipMalloc((void**)&d_v, mlp::P * sizeof(float)) == hipSuccess &&
+            hipMalloc((void**)&d_loss, (size_t)nblocks * sizeof(float)) == hipSuccess;
+  std::vector<float> zeros(mlp::P, 0.0f);
+  ok = ok && hipMemcpyAsync(d_m, zeros.data(), mlp::P * sizeof(float), hipMemcpyHostToDevice, stream) == hipSuccess &&
+       hipMemcpyAsync(d_v, zeros.data(), mlp::P * sizeof(float), hipMemcpyHostToDevice, stream) == hipSuccess &&
+       mlp::prepare_grad() == hipSuccess;
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::mt19937 rng(seed);
+  const double beta1 = 0.9, beta2 = 0.999;
+  long long step = 0;
+  std::vector<float> lossv(nblocks);
+  for (int ep = 0; ok && ep < epochs; ep++) {
+    std::shuffle(order.begin(), order.end(), rng);
+    ok = hipMemcpyAsync(d_idx, order.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, stream) == hipSuccess &&
+         hipStreamSynchronize(stream) == hipSuccess;
+    double sse = 0;
+    for (int base = 0; ok && base < n; base += mb) {
+      const int end = std::min(n, base + mb);
+      step++;
+      const float c1 = (float)(1.0 / (1.0 - std::pow(beta1, (double)step)));
+      const float c2 = (float)(1.0 / (1.0 - std::pow(beta2, (double)step)));
+      ok = mlp::launch_grad(params, x, y, d_idx, base, end, 1.0f / (float)(end - base), nblocks, d_part, d_loss,
+                            stream) == hipSuccess &&
+           mlp::launch_adam(d_part, nblocks, params, d_m, d_v, lr, (float)beta1, (float)beta2, c1, c2, 1e-8f,
+                            stream) == hipSuccess;
+      if (ok && epoch_loss) {
+        ok = hipMemcpyAsync(lossv.data(), d_loss, nblocks * sizeof(float), hipMemcpyDeviceToHost, stream) ==
+                 hipSuccess &&
+             hipStreamSynchronize(stream) == hipSuccess;
+        for (float l : lossv) sse += l;
+      }
+    }
+    if (epoch_loss) epoch_loss[ep] = (float)(sse / n);
+  }
+  ok = ok && hipStreamSynchronize(stream) == hipSuccess;
+  if (d_idx) (void)hipFree(d_idx);
+  if (d_part) (void)hipFree(d_part);
+  if (d_m) (void)hipFree(d_m);
+  if (d_v) (void)hipFree(d_v);
+  if (d_loss) (void)hipFree(d_loss);
+  if (!ok) {
+    (void)hipGetLastError();
+    return LIBRETT_C_INTERNAL_ERROR;
+  }
+  return LIBRETT_C_SUCCESS;
+}
+
+void librett_c_mlp_host_forward(const float* params, const float* x, int n, float* out) {
+  mlp::host_forward(params, x, n, out);
+}
+
+int librett_c_mlp_host_grad(const float* params, const float* x, const float* y, const int* idx, int count,
+                            float inv_m, float* grad, float* loss) {
+  if (!params || !x || !y || !idx || !grad || !loss || count < 0 || count > mlp::S) return LIBRETT_C_INVALID_PARAMETER;
+  mlp::host_grad(params, x, y, idx, count, inv_m, grad, loss);
+  return LIBRETT_C_SUCCESS;
+}
+
+void librett_c_mlp_host_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr,
+                             float beta1, float beta2, float c1, float c2, float eps) {
+  mlp::host_adam(partial, nparts, params, m, v, lr, beta1, beta2, c1, c2, eps);
+}
+
+}  // extern "C"

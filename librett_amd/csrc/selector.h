@@ -1,0 +1,51 @@
+// Learned plan selector: features of (problem, candidate) pairs, the MLP's
+// HIP kernels and host replays, and the selector state librettPlan consults.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "planner.h"
+
+namespace lrt {
+
+namespace mlp {
+
+hipError_t launch_forward(const float* params, const float* xs, int n, float* out, hipStream_t stream);
+hipError_t prepare_grad();
+hipError_t launch_grad(const float* params, const float* xs, const float* ys, const int* idx, int base, int end,
+                       float inv_m, int nblocks, float* partial, float* loss, hipStream_t stream);
+hipError_t launch_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr, float beta1,
+                       float beta2, float c1, float c2, float eps, hipStream_t stream);
+
+void host_forward(const float* params, const float* xs, int n, float* out);
+void host_grad(const float* params, const float* xs, const float* ys, const int* idx, int count, float inv_m,
+               float* grad, float* loss);
+void host_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr, float beta1,
+               float beta2, float c1, float c2, float eps);
+
+}  // namespace mlp
+
+// Largest reduced rank the features describe; larger problems use the
+// analytic model.
+constexpr int kSelMaxRank = 7;
+
+// Features per (problem, candidate) pair (the MLP's input width).
+constexpr int kSelFeatures = 48;
+
+// Raw (unnormalised) features of candidate c of problem `full`; `reduced`
+// is reduce_ranks(full). Writes mlp::F floats.
+void selector_features(const Problem& full, const Problem& reduced, const Candidate& c, float* f);
+
+// Index of the candidate the learned selector picks, or -1 when no weights
+// are loaded, the problem is outside the features' domain or inference
+// fails (the caller then uses the analytic model). Runs the HIP inference
+// kernel on the current device.
+int selector_choose(const std::vector<Candidate>& cands, const Problem& full, int device);
+
+// Generation of the loaded weights (0: none); part of the plan-cache key.
+uint64_t selector_generation();
+
+}  // namespace lrt

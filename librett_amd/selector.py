@@ -1,0 +1,331 @@
+"""Learned plan selector: data collection, training and evaluation.
+
+The MLP (2 x 64, FP32) scores every candidate plan of a problem from the
+features the library computes (``librett_c_selector_features``: reduced
+rank, log-dims, permutation, element size, the candidate's kernel shape and
+the analytic model's transaction counts and times). It is trained to
+predict ``log2(t / t_roof)`` -- the measured time of the candidate over the
+8 TB/s roofline time of the move -- on timings that ``librettPlanMeasure``'s
+own loop produces (``librett_c_measure_candidates``), with the library's HIP
+training kernels (``librett_c_mlp_train``). ``librettPlan`` then takes the
+candidate with the lowest score (``librett_c_selector_load``) and falls back
+to the analytic model when no weights are loaded.
+
+Command line (on a GPU)::
+
+    python -m librett_amd.selector collect --problems 20000 --seed 1 --out train.npz
+    python -m librett_amd.selector train --data train.npz --weights librett_amd/selector_weights.bin
+    python -m librett_amd.selector evaluate --problems 2000 --seed 2 --weights librett_amd/selector_weights.bin
+    python -m librett_amd.selector build --train 20000 --heldout 2000 --weights ... --report report.json
+
+The reference has no learned selector; its heuristic is choosePlanHeuristic
+(src/plan.cpp:721-789) over the GpuModel cycle counts (src/GpuModel.cpp).
+"""
+import argparse
+import json
+import os
+import struct
+import sys
+import time
+
+import numpy as np
+
+MAGIC = 0x4D54524C  # "LRTM"
+PEAK_BYTES_PER_S = 8.0e12
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_weights.bin")
+
+
+# ---------------------------------------------------------------------------
+# Weights file
+# ---------------------------------------------------------------------------
+
+def save_weights(path, params, mean, std):
+    from . import mlp_sizes
+    nf, nh, npar = mlp_sizes()
+    params = np.asarray(params, dtype="<f4")
+    assert params.size == npar and len(mean) == nf and len(std) == nf
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<4I", MAGIC, 1, nf, nh))
+        fh.write(np.asarray(mean, dtype="<f4").tobytes())
+        fh.write(np.asarray(std, dtype="<f4").tobytes())
+        fh.write(params.tobytes())
+
+
+def read_weights(path):
+    from . import mlp_sizes
+    nf, nh, npar = mlp_sizes()
+    raw = open(path, "rb").read()
+    magic, ver, f, h = struct.unpack("<4I", raw[:16])
+    if (magic, ver, f, h) != (MAGIC, 1, nf, nh):
+        raise ValueError("not a selector weights file: %s" % path)
+    a = np.frombuffer(raw[16:], dtype="<f4")
+    if a.size != 2 * nf + npar:
+        raise ValueError("truncated selector weights file: %s" % path)
+    return a[2 * nf:].copy(), a[:nf].copy(), a[nf:2 * nf].copy()
+
+
+def init_params(seed):
+    """He-initialised parameters (biases zero)."""
+    from . import mlp_sizes
+    nf, nh, npar = mlp_sizes()
+    rng = np.random.default_rng(seed)
+    w1 = rng.normal(0, np.sqrt(2.0 / nf), (nh, nf))
+    w2 = rng.normal(0, np.sqrt(2.0 / nh), (nh, nh))
+    w3 = rng.normal(0, np.sqrt(1.0 / nh), nh)
+    p = np.concatenate([w1.ravel(), np.zeros(nh), w2.ravel(), np.zeros(nh), w3, [0.0]]).astype(np.float32)
+    assert p.size == npar
+    return p
+
+
+# ---------------------------------------------------------------------------
+# Problems
+# ---------------------------------------------------------------------------
+
+def random_problem(rng, rank_lo=2, rank_hi=7, bytes_lo=1 << 20, bytes_hi=512 << 20, elems=(4, 8, 16)):
+    """Rank, element size and a random permutation drawn uniformly; the
+    volume log-uniform in [bytes_lo, bytes_hi] bytes, split over the dims by
+    a random log-partition (every dim >= 2)."""
+    elem = int(rng.choice(elems))
+    rank = int(rng.integers(rank_lo, rank_hi + 1))
+    target = float(np.exp(rng.uniform(np.log(bytes_lo), np.log(bytes_hi)))) / elem
+    w = rng.dirichlet(np.ones(rank))
+    dims = [max(2, int(round(target ** wi))) for wi in w]
+    # Trim the largest dims until the volume is within the range.
+    while int(np.prod(dims)) * elem > bytes_hi:
+        i = int(np.argmax(dims))
+        dims[i] = max(2, dims[i] // 2)
+    perm = [int(v) for v in rng.permutation(rank)]
+    return dims, perm, elem
+
+
+def problems(n, seed, **kw):
+    rng = np.random.default_rng(seed)
+    return [random_problem(rng, **kw) for _ in range(n)]
+
+
+# ---------------------------------------------------------------------------
+# Collection (GPU)
+# ---------------------------------------------------------------------------
+
+def collect(probs, stream, in_ptr, out_ptr, log=None):
+    """Features and measured times of every candidate of every problem."""
+    from . import measure_candidates, selector_features
+    recs = []
+    t0 = time.time()
+    last = t0
+    for k, (d, p, e) in enumerate(probs):
+        feats, mc = selector_features(d, p, e)
+        times = measure_candidates(d, p, e, stream, in_ptr, out_ptr)
+        n = min(len(feats), len(times))
+        recs.append({"dims": list(d), "perm": list(p), "elem": int(e), "feats": feats[:n], "times": times[:n],
+                     "model_choice": int(mc)})
+        if log and (time.time() - last > 20 or k + 1 == len(probs)):
+            last = time.time()
+            log("collected %d/%d problems (%.0f s)" % (k + 1, len(probs), last - t0))
+    return recs
+
+
+def save_records(path, recs):
+    off = np.cumsum([0] + [len(r["times"]) for r in recs])
+    np.savez_compressed(
+        path,
+        feats=np.concatenate([r["feats"] for r in recs]).astype(np.float32),
+        times=np.concatenate([r["times"] for r in recs]),
+        offsets=off,
+        model_choice=np.array([r["model_choice"] for r in recs]),
+        elem=np.array([r["elem"] for r in recs]),
+        problems=np.array([json.dumps([r["dims"], r["perm"]]) for r in recs]),
+    )
+
+
+def load_records(path):
+    z = np.load(path, allow_pickle=False)
+    off = z["offsets"]
+    recs = []
+    for i in range(len(off) - 1):
+        a, b = off[i], off[i + 1]
+        d, p = json.loads(str(z["problems"][i]))
+        recs.append({"dims": d, "perm": p, "elem": int(z["elem"][i]), "feats": z["feats"][a:b],
+                     "times": z["times"][a:b], "model_choice": int(z["model_choice"][i])})
+    return recs
+
+
+# ---------------------------------------------------------------------------
+# Training and evaluation
+# ---------------------------------------------------------------------------
+
+def targets(rec):
+    """log2(t / t_roof) of every candidate that ran (NaN where it did not)."""
+    nbytes = 2.0 * float(np.prod(rec["dims"])) * rec["elem"]
+    t = np.asarray(rec["times"], dtype=np.float64)
+    y = np.full(len(t), np.nan)
+    ok = t > 0
+    y[ok] = np.log2(t[ok] / (nbytes / PEAK_BYTES_PER_S))
+    return y
+
+
+def dataset(recs):
+    xs, ys = [], []
+    for r in recs:
+        y = targets(r)
+        ok = np.isfinite(y)
+        xs.append(r["feats"][ok])
+        ys.append(y[ok])
+    return np.concatenate(xs).astype(np.float32), np.concatenate(ys).astype(np.float32)
+
+
+def normaliser(x):
+    mean = x.mean(axis=0)
+    std = x.std(axis=0)
+    std[std < 1e-6] = 1.0
+    return mean.astype(np.float32), std.astype(np.float32)
+
+
+def train(recs, stream, epochs=60, lr=2e-3, minibatch=2048, seed=1, log=None):
+    """Trains on the GPU; returns (params, mean, std, per-epoch loss)."""
+    from . import hip, mlp_train
+    x, y = dataset(recs)
+    mean, std = normaliser(x)
+    xn = ((x - mean) / std).astype(np.float32)
+    p = init_params(seed)
+    dx, dy, dp = hip.DeviceBuffer(xn.nbytes), hip.DeviceBuffer(y.nbytes), hip.DeviceBuffer(p.nbytes)
+    hip.htod(dx.ptr, xn)
+    hip.htod(dy.ptr, y)
+    hip.htod(dp.ptr, p)
+    losses = []
+    chunk = 10
+    for e0 in range(0, epochs, chunk):
+        n_ep = min(chunk, epochs - e0)
+        losses += list(mlp_train(dx.ptr, dy.ptr, len(y), dp.ptr, n_ep, lr, minibatch, seed + e0, stream))
+        if log:
+            log("epoch %d: mse %.4f" % (e0 + n_ep, losses[-1]))
+    hip.stream_sync(stream)
+    hip.dtoh(p, dp.ptr)
+    for b in (dx, dy, dp):
+        b.free()
+    return p, mean, std, np.array(losses)
+
+
+def scores(params, mean, std, recs, stream):
+    """MLP scores of every candidate, computed by the HIP inference kernel."""
+    from . import hip, mlp_forward
+    x = np.concatenate([r["feats"] for r in recs]).astype(np.float32)
+    xn = ((x - mean) / std).astype(np.float32)
+    out = np.zeros(len(xn), dtype=np.float32)
+    p = np.asarray(params, dtype=np.float32)
+    dx, dp, do = hip.DeviceBuffer(max(4, xn.nbytes)), hip.DeviceBuffer(p.nbytes), hip.DeviceBuffer(max(4, out.nbytes))
+    hip.htod(dx.ptr, xn)
+    hip.htod(dp.ptr, p)
+    mlp_forward(dp.ptr, dx.ptr, len(xn), do.ptr, stream)
+    hip.stream_sync(stream)
+    hip.dtoh(out, do.ptr)
+    for b in (dx, dp, do):
+        b.free()
+    res, o = [], 0
+    for r in recs:
+        res.append(out[o:o + len(r["times"])])
+        o += len(r["times"])
+    return res
+
+
+def evaluate(recs, sc):
+    """Slowdown of the selector's and the analytic model's choice versus the
+    fastest measured candidate, per problem."""
+    sel, ana = [], []
+    for r, s in zip(recs, sc):
+        t = np.asarray(r["times"], dtype=np.float64)
+        ok = t > 0
+        if not ok.any():
+            continue
+        best = t[ok].min()
+        s = np.where(ok, s, np.inf)
+        i = int(np.argmin(s))
+        sel.append(t[i] / best)
+        m = r["model_choice"]
+        ana.append(t[m] / best if 0 <= m < len(t) and t[m] > 0 else np.nan)
+    return np.array(sel), np.array(ana)
+
+
+def summary(slow):
+    slow = slow[np.isfinite(slow)]
+    return {"mean": float(slow.mean()), "geomean": float(np.exp(np.log(slow).mean())),
+            "p50": float(np.median(slow)), "p90": float(np.percentile(slow, 90)),
+            "p99": float(np.percentile(slow, 99)), "max": float(slow.max()),
+            "within_5pct": float((slow <= 1.05).mean()), "exact": float((slow <= 1.0 + 1e-9).mean()),
+            "problems": int(len(slow))}
+
+
+def _buffers(max_bytes):
+    from . import hip
+    a, b = hip.DeviceBuffer(max_bytes), hip.DeviceBuffer(max_bytes)
+    hip.memset(a.ptr, 0, max_bytes)
+    hip.memset(b.ptr, 0, max_bytes)
+    return a, b
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m librett_amd.selector")
+    ap.add_argument("cmd", choices=["collect", "train", "evaluate", "build"])
+    ap.add_argument("--train", type=int, default=20000, help="build: training problems (seed --seed)")
+    ap.add_argument("--heldout", type=int, default=2000, help="build: held-out problems (seed --seed + 1000)")
+    ap.add_argument("--report", default=None, help="build: JSON report path")
+    ap.add_argument("--problems", type=int, default=2000)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--out", default="selector_data.npz")
+    ap.add_argument("--data", default=None)
+    ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    a = ap.parse_args(argv)
+    from . import hip
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    stream = hip.stream_create()
+    if a.cmd == "build":
+        A, B = _buffers(512 << 20)
+        t0 = time.time()
+        tr = collect(problems(a.train, a.seed), stream, A.ptr, B.ptr, log)
+        t1 = time.time()
+        p, mean, std, losses = train(tr, stream, a.epochs, a.lr, log=log)
+        t2 = time.time()
+        save_weights(a.weights, p, mean, std)
+        ho = collect(problems(a.heldout, a.seed + 1000), stream, A.ptr, B.ptr, log)
+        s_tr, a_tr = evaluate(tr, scores(p, mean, std, tr, stream))
+        s_ho, a_ho = evaluate(ho, scores(p, mean, std, ho, stream))
+        rep = {"train_problems": len(tr), "heldout_problems": len(ho), "samples": int(sum(len(r["times"]) for r in tr)),
+               "epochs": a.epochs, "lr": a.lr, "final_mse": float(losses[-1]), "collect_s": t1 - t0,
+               "train_s": t2 - t1, "seed": a.seed,
+               "train": {"selector": summary(s_tr), "analytic": summary(a_tr)},
+               "heldout": {"selector": summary(s_ho), "analytic": summary(a_ho)}}
+        print(json.dumps(rep))
+        if a.report:
+            with open(a.report, "w") as fh:
+                json.dump(rep, fh, indent=1)
+        if a.out:
+            save_records(a.out, ho)
+    elif a.cmd == "collect":
+        A, B = _buffers(512 << 20)
+        recs = collect(problems(a.problems, a.seed), stream, A.ptr, B.ptr, log)
+        save_records(a.out, recs)
+    elif a.cmd == "train":
+        recs = load_records(a.data)
+        p, mean, std, losses = train(recs, stream, a.epochs, a.lr, log=log)
+        save_weights(a.weights, p, mean, std)
+        sel, ana = evaluate(recs, scores(p, mean, std, recs, stream))
+        print(json.dumps({"train": {"selector": summary(sel), "analytic": summary(ana),
+                                    "final_mse": float(losses[-1])}}))
+    else:
+        if a.data:
+            recs = load_records(a.data)
+        else:
+            A, B = _buffers(512 << 20)
+            recs = collect(problems(a.problems, a.seed), stream, A.ptr, B.ptr, log)
+        p, mean, std = read_weights(a.weights)
+        sel, ana = evaluate(recs, scores(p, mean, std, recs, stream))
+        print(json.dumps({"heldout": {"selector": summary(sel), "analytic": summary(ana)}}))
+
+
+if __name__ == "__main__":
+    main()
