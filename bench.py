@@ -49,11 +49,15 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--measure", action="store_true", help="choose plans with librettPlanMeasure")
+    ap.add_argument("--analytic", action="store_true",
+                    help="librettPlan with the analytic model only (no learned selector)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=5)
-    ap.add_argument("--workload", default="large", choices=["large", "batched", "serving"],
+    ap.add_argument("--workload", default="large", choices=["large", "batched", "serving", "selector"],
                     help="large: the seven ~256 MiB cases (default); batched: 4096 uneven problems; "
-                         "serving: Poisson request stream through the continuous-batching engine")
+                         "serving: Poisson request stream through the continuous-batching engine; "
+                         "selector: learned plan selector vs analytic model on held-out problems")
+    ap.add_argument("--heldout", type=int, default=2000, help="selector workload: held-out problems")
     ap.add_argument("--requests", type=int, default=20000, help="serving workload: arrivals per rate")
     ap.add_argument("--arena-gb", type=float, default=64.0, help="serving workload: device arena budget (GiB)")
     ap.add_argument("--problems", type=int, default=4096, help="batched workload: number of problems")
@@ -73,6 +77,12 @@ def main():
         return bench_batched(args, stream)
     if args.workload == "serving":
         return bench_serving(args, stream)
+    if args.workload == "selector":
+        return bench_selector(args, stream)
+    if args.analytic:
+        L.selector_set(None, None, None)
+    selection = ("librettPlanMeasure" if args.measure else
+                 "learned selector (librettPlan)" if L.selector_generation() else "gfx950 analytic model (librettPlan)")
 
     # One random source buffer, uploaded as the input of every case.
     max_bytes = max(int(np.prod(d)) * e for d, _, e in CASES)
@@ -208,7 +218,7 @@ def main():
             "workload": "large single transposes ~256 MiB each (7 named cases, f32 and f64)",
             "cases": len(cases),
             "bytes_per_step": step_bytes,
-            "plan_selection": "librettPlanMeasure" if args.measure else "gfx950 model (librettPlan)",
+            "plan_selection": selection,
         },
         "d2d_copy_gbs": round(d2d_gbs, 1),
         "frac_of_d2d": round(value / d2d_gbs, 3),
@@ -227,6 +237,42 @@ def main():
         },
         "cpu_baseline": cpu,
         "cases": per_case,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_selector(args, stream):
+    """Learned plan selector: every candidate of each held-out problem timed
+    with librettPlanMeasure's loop; the slowdown of the selector's choice and
+    of the analytic model's choice versus the fastest candidate."""
+    import librett_amd as L
+    from librett_amd import selector as S
+    if not L.selector_generation():
+        raise SystemExit("no selector weights loaded (librett_amd/selector_weights.bin)")
+    A, B = S._buffers(512 << 20)
+    t0 = time.perf_counter()
+    recs = S.collect(S.problems(args.heldout, 1001), stream, A.ptr, B.ptr, log)
+    coll = time.perf_counter() - t0
+    p, mean, std = S.read_weights(S.DEFAULT_WEIGHTS)
+    t1 = time.perf_counter()
+    sc = S.scores(p, mean, std, recs, stream)
+    infer = time.perf_counter() - t1
+    sel, ana = S.evaluate(recs, sc)
+    ss, sa = S.summary(sel), S.summary(ana)
+    ncand = sum(len(r["times"]) for r in recs)
+    log("selector: mean slowdown %.4f (p50 %.3f p90 %.3f p99 %.3f, %.1f%% within 5%%); analytic %.4f (p50 %.3f "
+        "p90 %.3f p99 %.3f, %.1f%% within 5%%); %d problems, %d candidates timed in %.0f s; inference %.1f ms"
+        % (ss["mean"], ss["p50"], ss["p90"], ss["p99"], 100 * ss["within_5pct"], sa["mean"], sa["p50"], sa["p90"],
+           sa["p99"], 100 * sa["within_5pct"], len(recs), ncand, coll, infer * 1e3))
+    line = {
+        "metric": "plan selection: mean slowdown of the chosen plan vs the exhaustive best (held-out problems)",
+        "value": round(ss["mean"], 4), "unit": "x", "n_gpus": 1, "steps": len(recs), "warmup": 0,
+        "ms_per_step": round(coll / max(1, len(recs)) * 1e3, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (MLP); f32/f64/c128 opaque moves timed", "data": "synthetic",
+        "config": {"workload": "learned selector: %d held-out problems, rank 2-7, 1 MB-512 MB, f32/f64/c128"
+                               % len(recs), "seed": 1001, "candidates_timed": ncand},
+        "selector": ss, "analytic": sa, "inference_ms_all_candidates": round(infer * 1e3, 2),
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 

@@ -247,11 +247,13 @@ const char* librett_c_version(void);
  * analytic model only, src/plan.cpp:721-789). A 2 x 64 FP32 MLP scores
  * every candidate plan of a problem from LIBRETT_C_SELECTOR_FEATURES
  * features; librettPlan / librett_c_plan take its lowest score when
- * weights are loaded (librett_c_selector_load / _set, or the file named by
- * LIBRETT_SELECTOR_WEIGHTS) and the analytic model otherwise, or when the
- * reduced rank exceeds 7.
+ * weights are loaded and the analytic model otherwise, or when the reduced
+ * rank exceeds 7. Weights load on first use from the file named by
+ * LIBRETT_SELECTOR_WEIGHTS (empty or "none": analytic model only), else from
+ * selector_weights.bin beside the library; librett_c_selector_load / _set
+ * replace them at run time.
  * ------------------------------------------------------------------------ */
-#define LIBRETT_C_SELECTOR_FEATURES 48
+#define LIBRETT_C_SELECTOR_FEATURES 50
 
 /* Input features, hidden width and parameter count of the MLP. */
 void librett_c_mlp_sizes(int* nfeat, int* hidden, int* nparams);
@@ -273,19 +275,19 @@ int librett_c_selector_features(int rank, const int* dim, const int* permutation
 /* Candidate the loaded selector picks on the current GPU (-1: analytic). */
 int librett_c_selector_choose(int rank, const int* dim, const int* permutation, size_t sizeofType, int* index);
 
-/* Loads weights: nparams floats in the layout W1[64][48], b1[64], W2[64][64],
+/* Loads weights: nparams floats in the layout W1[64][50], b1[64], W2[64][64],
  * b2[64], w3[64], b3, with per-feature mean and std used to normalise the
  * inputs. params NULL unloads. */
 int librett_c_selector_set(const float* params, int nparams, const float* mean, const float* std, int nfeat);
 
-/* Loads a weights file: uint32 magic 0x4D54524C, version 1, 48, 64, then
- * float32 mean[48], std[48], params[nparams] (little-endian). */
+/* Loads a weights file: uint32 magic 0x4D54524C, version 1, 50, 64, then
+ * float32 mean[50], std[50], params[nparams] (little-endian). */
 int librett_c_selector_load(const char* path);
 
 /* Generation of the loaded weights (0: none). */
 unsigned long long librett_c_selector_generation(void);
 
-/* MLP kernels on device pointers. forward: out[n] = MLP(x[n][48]).
+/* MLP kernels on device pointers. forward: out[n] = MLP(x[n][50]).
  * grad: workgroup b (of nblocks) writes the gradient of samples
  * idx[base + 128 b .. min(end, base + 128 (b+1))) of sum (out - y)^2 * inv_m
  * to partial[b][nparams] and its squared-error sum to loss[b].

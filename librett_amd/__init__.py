@@ -504,7 +504,7 @@ def measure_candidates(dim, permutation, sizeofType, stream, idata, odata, max_c
 
 
 def selector_features(dim, permutation, sizeofType, device=None, max_count=1024):
-    """(features [n, 48] float32, analytic model's choice). device None: the current GPU."""
+    """(features [n, 50] float32, analytic model's choice). device None: the current GPU."""
     import numpy as np
     nf = mlp_sizes()[0]
     f = np.zeros((max_count, nf), dtype=np.float32)

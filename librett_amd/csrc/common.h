@@ -159,8 +159,24 @@ struct PackedArgs {
   int split_in_stride;   // input stride of the split dim
   int split_out_stride;  // output stride of the split dim
   int vol_outer;
-  int outer_per_block;   // contiguous outer indices per block row
+  int outer_per_block;   // order 0: contiguous outer indices per block row
+  // Work order. 0: block (x, y) takes chunk x and outer indices
+  // [y * outer_per_block, (y + 1) * outer_per_block). 1 (XCD-interleaved):
+  // the linear block id is remapped so that the blocks one XCD runs at the
+  // same time take consecutive outer indices of one chunk; a block then
+  // strides by gridDim.y. Adjacent outer indices that share cache lines are
+  // then fetched once into that XCD's L2.
+  int order;
+  int xcds;              // XCDs the dispatcher deals workgroups to, round-robin
   OuterMap outer;
 };
+
+// Order-1 mapping: linear block b of G -> logical block; blocks b, b + x,
+// b + 2x, ... (one XCD, dispatched together) get consecutive logical ids.
+__host__ __device__ inline int xcd_logical(int b, int G, int xcds) {
+  const int q = G / xcds, r = G - (G / xcds) * xcds;
+  const int x = b - (b / xcds) * xcds;
+  return x * q + (x < r ? x : r) + b / xcds;
+}
 
 }  // namespace lrt

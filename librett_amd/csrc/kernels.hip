@@ -60,10 +60,22 @@ __global__ __launch_bounds__(1024) void k_packed(PackedArgs a, const void* __res
   const int nt = blockDim.x;
   const int t = threadIdx.x;
   typename Ops::Pre p;
-  Ops::prepare(a, blockIdx.x, t, nt, p);
-  const int o0 = blockIdx.y * a.outer_per_block;
-  const int o1 = min(a.vol_outer, o0 + a.outer_per_block);
-  for (int o = o0; o < o1; o++) {
+  int split, o0, o1, step;
+  if (a.order == 0) {
+    split = blockIdx.x;
+    o0 = blockIdx.y * a.outer_per_block;
+    o1 = min(a.vol_outer, o0 + a.outer_per_block);
+    step = 1;
+  } else {
+    const int G = gridDim.x * gridDim.y;
+    const int L = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, G, a.xcds);
+    split = L / gridDim.y;
+    o0 = L - split * gridDim.y;
+    o1 = a.vol_outer;
+    step = gridDim.y;
+  }
+  Ops::prepare(a, split, t, nt, p);
+  for (int o = o0; o < o1; o += step) {
     int off_in, off_out;
     outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
     Ops::load(p, off_in, t, nt, src, lds);
@@ -152,19 +164,30 @@ static void replay_packed(const Candidate& c, const void* in, void* out) {
   const int nt = c.block;
   std::vector<E> lds(c.lds_bytes / B + 1);
   std::vector<typename Ops::Pre> pre(nt);
-  for (int bx = 0; bx < c.grid_x; bx++) {
-    for (int t = 0; t < nt; t++) Ops::prepare(a, bx, t, nt, pre[t]);
-    for (int by = 0; by < c.grid_y; by++) {
-      const int o0 = by * a.outer_per_block;
-      const int o1 = std::min(a.vol_outer, o0 + a.outer_per_block);
-      for (int o = o0; o < o1; o++) {
-        int off_in, off_out;
-        outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
-        for (int t = 0; t < nt; t++)
-          Ops::load(pre[t], off_in, t, nt, static_cast<const E*>(in), lds.data());
-        for (int t = 0; t < nt; t++)
-          Ops::store(pre[t], off_out, t, nt, static_cast<E*>(out), lds.data());
-      }
+  const int G = c.grid_x * c.grid_y;
+  for (int b = 0; b < G; b++) {
+    const int bx = b % c.grid_x, by = b / c.grid_x;
+    int split, o0, o1, step;
+    if (a.order == 0) {
+      split = bx;
+      o0 = by * a.outer_per_block;
+      o1 = std::min(a.vol_outer, o0 + a.outer_per_block);
+      step = 1;
+    } else {
+      const int L = xcd_logical(b, G, a.xcds);
+      split = L / c.grid_y;
+      o0 = L - split * c.grid_y;
+      o1 = a.vol_outer;
+      step = c.grid_y;
+    }
+    for (int t = 0; t < nt; t++) Ops::prepare(a, split, t, nt, pre[t]);
+    for (int o = o0; o < o1; o += step) {
+      int off_in, off_out;
+      outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
+      for (int t = 0; t < nt; t++)
+        Ops::load(pre[t], off_in, t, nt, static_cast<const E*>(in), lds.data());
+      for (int t = 0; t < nt; t++)
+        Ops::store(pre[t], off_out, t, nt, static_cast<E*>(out), lds.data());
     }
   }
 }

@@ -245,9 +245,9 @@ std::string Candidate::describe() const {
       break;
     case kPacked:
     case kPackedSplit:
-      snprintf(buf, sizeof(buf), "%s(%s) Mm %d Mk %d volMmk %d split %d/%d nreg %d block %d grid %dx%d lds %d", n,
+      snprintf(buf, sizeof(buf), "%s(%s) Mm %d Mk %d volMmk %d split %d/%d nreg %d block %d grid %dx%d lds %d%s", n,
                source, split.sizeMm, split.sizeMk, split.volMmk, split.numSplit, split.splitRank, nreg, block, grid_x,
-               grid_y, lds_bytes);
+               grid_y, lds_bytes, packed.order ? " xcd" : "");
       break;
     default:
       snprintf(buf, sizeof(buf), "%s", n);
@@ -524,7 +524,15 @@ void push_packed(const Problem& p, const char* source, const Split& s, int nt, i
   const int gy = (int)std::max<long long>(1, std::min<long long>(vol_outer, want));
   a.outer_per_block = (vol_outer + gy - 1) / gy;
   c.grid_y = (vol_outer + a.outer_per_block - 1) / a.outer_per_block;
+  a.order = 0;
+  a.xcds = std::max(1, dev.cu_count / 32);  // 32 CUs per XCD on gfx950
   out.push_back(c);
+  // The XCD-interleaved order of the same plan (the batched engine has its
+  // own unit order).
+  if (!dev.batch_mode && c.grid_y > 1 && a.outer_per_block > 1) {
+    c.packed.order = 1;
+    out.push_back(c);
+  }
 }
 
 void add_packed(const Problem& p, const char* source, const DeviceInfo& dev, std::vector<Candidate>& out) {
@@ -851,7 +859,9 @@ void model_packed(Candidate& c, const DeviceInfo& dev) {
   // to stay in L2; past that footprint it fades out.
   const double resident = std::min((double)c.grid_x * c.grid_y, (double)dev.cu_count * std::max(1, c.active_blocks_per_cu));
   const double footprint = resident * 128.0 * (R1 + W1) / std::max(1.0, rounds);
-  const double keep = std::min(1.0, dev.l2_bytes / std::max(1.0, footprint));
+  // In the XCD-interleaved order the blocks sharing lines run together on
+  // one XCD, so the reuse does not depend on the lines surviving in L2.
+  const double keep = a.order == 1 ? 1.0 : std::min(1.0, dev.l2_bytes / std::max(1.0, footprint));
   c.model.read_lines = (R * keep + R1 * (1 - keep)) * scale;
   c.model.write_lines = (W * keep + W1 * (1 - keep)) * scale;
   c.model.partial_lines = P * scale;

@@ -5,6 +5,7 @@
 // (choosePlanHeuristic, src/plan.cpp:721-789); its librettPlanMeasure
 // (src/librett.cpp:224-343) times every candidate, which is where the
 // selector's training labels come from.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,6 +16,7 @@
 #include <mutex>
 #include <numeric>
 #include <random>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -90,13 +92,34 @@ int load_weights(const char* path) {
   return set_weights(params.data(), mlp::P, mean.data(), stdv.data(), mlp::F);
 }
 
-// Weights named by LIBRETT_SELECTOR_WEIGHTS are loaded on first use.
+// Directory of the shared library that holds this code.
+std::string library_dir() {
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void*>(&library_dir), &info) == 0 || !info.dli_fname) return std::string();
+  std::string p(info.dli_fname);
+  const size_t slash = p.find_last_of('/');
+  return slash == std::string::npos ? std::string(".") : p.substr(0, slash);
+}
+
+// On first use: the weights file named by LIBRETT_SELECTOR_WEIGHTS, or
+// selector_weights.bin beside the library. LIBRETT_SELECTOR_WEIGHTS set to
+// an empty string or "none" keeps the analytic model.
 void load_from_env() {
   static std::once_flag once;
   std::call_once(once, [] {
     const char* p = std::getenv("LIBRETT_SELECTOR_WEIGHTS");
-    if (p && *p && load_weights(p) != LIBRETT_C_SUCCESS)
-      std::fprintf(stderr, "librett: cannot load selector weights from %s; using the analytic model\n", p);
+    if (p) {
+      if (!*p || std::strcmp(p, "none") == 0) return;
+      if (load_weights(p) != LIBRETT_C_SUCCESS)
+        std::fprintf(stderr, "librett: cannot load selector weights from %s; using the analytic model\n", p);
+      return;
+    }
+    const std::string def = library_dir() + "/selector_weights.bin";
+    FILE* fp = std::fopen(def.c_str(), "rb");
+    if (!fp) return;
+    std::fclose(fp);
+    if (load_weights(def.c_str()) != LIBRETT_C_SUCCESS)
+      std::fprintf(stderr, "librett: cannot load selector weights from %s; using the analytic model\n", def.c_str());
   });
 }
 
@@ -141,6 +164,8 @@ void selector_features(const Problem& full, const Problem& red, const Candidate&
   const double t_roof = std::max(1.0, (double)c.bytes) / 8.0e12;
   f[46] = lg(m.predicted_s / t_roof);
   f[47] = lg(m.t_stream / t_roof);
+  f[48] = (float)((c.method == kPacked || c.method == kPackedSplit) ? c.packed.order : 0);
+  f[49] = std::strcmp(c.source, "full") == 0 ? 1.0f : 0.0f;
 }
 
 uint64_t selector_generation() {

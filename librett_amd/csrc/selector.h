@@ -33,7 +33,7 @@ void host_adam(const float* partial, int nparts, float* params, float* m, float*
 constexpr int kSelMaxRank = 7;
 
 // Features per (problem, candidate) pair (the MLP's input width).
-constexpr int kSelFeatures = 48;
+constexpr int kSelFeatures = 50;
 
 // Raw (unnormalised) features of candidate c of problem `full`; `reduced`
 // is reduce_ranks(full). Writes mlp::F floats.

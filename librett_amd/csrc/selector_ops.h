@@ -13,7 +13,7 @@
 namespace lrt {
 namespace mlp {
 
-constexpr int F = 48;  // features per (problem, candidate)
+constexpr int F = 50;  // features per (problem, candidate)
 constexpr int H = 64;  // width of both hidden layers
 
 // Parameter layout (floats): W1[H][F], b1[H], W2[H][H], b2[H], w3[H], b3.

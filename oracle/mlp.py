@@ -5,7 +5,7 @@ calls it).
 
 The reference has no learned selector (it ranks candidates with its
 analytic model, ``src/plan.cpp:721-789``), so this restates the network the
-library defines: inputs x (48 normalised features), two ReLU layers of
+library defines: inputs x (50 normalised features), two ReLU layers of
 width 64, one linear output,
 
     z1 = W1 x + b1,  a1 = relu(z1)
@@ -17,7 +17,7 @@ with the loss ``inv_m * sum_s (out_s - y_s)^2`` and the Adam update
 beta1^t)``, ``c2 = 1/(1 - beta2^t)`` applied as
 ``p -= lr * (m c1) / (sqrt(v c2) + eps)``.
 
-Parameter vector layout: W1[64][48], b1[64], W2[64][64], b2[64], w3[64], b3.
+Parameter vector layout: W1[64][50], b1[64], W2[64][64], b2[64], w3[64], b3.
 
 Pinned in ``tests/test_selector_cpu.py`` by: a hand-worked two-unit network
 (forward), central finite differences of the loss (backward), and Adam's
@@ -25,7 +25,7 @@ closed-form first step ``-lr * g / (|g| + eps)`` (update).
 """
 import numpy as np
 
-F = 48
+F = 50
 H = 64
 P = H * F + H + H * H + H + H + 1
 

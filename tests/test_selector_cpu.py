@@ -20,7 +20,7 @@ def rand_params(seed, scale=1.0):
 
 
 def test_sizes_match_restatement():
-    assert (NF, NH, NP) == (M.F, M.H, M.P) == (48, 64, 7361)
+    assert (NF, NH, NP) == (M.F, M.H, M.P) == (50, 64, 64 * 50 + 64 + 64 * 64 + 64 + 64 + 1)
 
 
 def test_forward_hand_worked():
