@@ -58,11 +58,18 @@ def main():
                          "serving: Poisson request stream through the continuous-batching engine; "
                          "selector: learned plan selector vs analytic model on held-out problems")
     ap.add_argument("--heldout", type=int, default=2000, help="selector workload: held-out problems")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "RUN_LOG"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
+                         "'bench.py --steps 1 --warmup 0 --no-cpu' (RUN_LOG: that run's stderr, for the plans) "
+                         "into HBM traffic per named case; no GPU needed")
+    ap.add_argument("--pmc-json", default=None, help="with --pmc-summary: also write the summary as JSON here")
     ap.add_argument("--requests", type=int, default=20000, help="serving workload: arrivals per rate")
     ap.add_argument("--arena-gb", type=float, default=64.0, help="serving workload: device arena budget (GiB)")
     ap.add_argument("--problems", type=int, default=4096, help="batched workload: number of problems")
     ap.add_argument("--tune", type=int, default=3, help="batched workload: autotuner candidates per problem")
     args = ap.parse_args()
+    if args.pmc_summary:
+        return pmc_summary(*args.pmc_summary, json_path=args.pmc_json)
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X and has no "
               "multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -153,6 +160,14 @@ def main():
         kb[2] += args.steps
     dom_name, (dom_ms, dom_bytes, dom_launches) = max(by_kernel.items(), key=lambda kv: kv[1][0])
     dom_gbs = dom_bytes / (dom_ms * 1e-3) / 1e9
+    # HBM bytes per launch of the dominant kernel from the committed counter
+    # summary, when it was taken with the same plan.
+    dom_traffic = None
+    for c in cases:
+        if c["info"]["method_name"] + ":" + c["info"]["desc"].split(" grid")[0] == dom_name:
+            t = measured_traffic(c["dim"], c["perm"], c["elem"])
+            if t and t.get("plan") == c["info"]["desc"]:
+                dom_traffic = t["traffic_bytes"]
 
     # Ceilings measured the same way: each transpose of a step replaced by a
     # copy of the same bytes between the same buffers, for the same steps.
@@ -231,7 +246,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": dom_traffic,
+            "traffic_over_algorithmic": round(dom_traffic / (dom_bytes / dom_launches), 4) if dom_traffic else None,
             "launches": dom_launches,
             "avg_launch_ms": round(dom_ms / dom_launches, 4),
         },
@@ -488,6 +504,80 @@ def serving_catalog(kw, n):
         d, p, _ = W.random_problem(rng, rank, rank, max(2.0, nbytes / e), max(2.0, nbytes / e), [e])
         out.append((d, p, e))
     return out
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_named_large_cases.json")
+
+
+def _pmc_rows(d):
+    import csv
+    import glob
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no counter_collection.csv under %s" % d)
+    rows = list(csv.DictReader(open(files[0])))
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    return rows
+
+
+def pmc_summary(fetch_dir, write_dir, run_log, json_path=None):
+    """HBM traffic of the named cases from two counter runs of
+    'bench.py --steps 1 --warmup 0 --no-cpu'. The counters are scaled by
+    what they read for the streaming copy kernel of the same run, whose
+    traffic is known (Grid_Size 16-byte words read and written)."""
+    fr, wr = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
+    if len(fr) != len(wr):
+        raise SystemExit("the two runs dispatched different kernel counts")
+    fetch = [float(r["Counter_Value"]) * 1024 for r in fr]   # KiB -> bytes
+    write = [float(r["Counter_Value"]) * 1024 for r in wr]
+    names = [r["Kernel_Name"] for r in fr]
+    grids = [int(r["Grid_Size"]) for r in fr]
+    cal_r, cal_w = [], []
+    for n, g, f, w in zip(names, grids, fetch, write):
+        if "k_devcopy" in n:
+            cal_r.append(g * 16 / f)
+            cal_w.append(g * 16 / w)
+    if not cal_r:
+        raise SystemExit("no copy-kernel dispatches to calibrate against")
+    fr_k, fw_k = float(np.median(cal_r)), float(np.median(cal_w))
+    transposes = [i for i, n in enumerate(names) if any(k in n for k in ("k_tiled", "k_copy<", "k_packed"))]
+    plans = [ln.split(" -> ", 1)[1].rsplit(" (", 1)[0].strip() for ln in open(run_log) if ln.startswith("plan ")]
+    out = {"fetch_factor": round(fr_k, 4), "write_factor": round(fw_k, 4), "cases": []}
+    lines = ["# HBM traffic of the named large cases (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)", "",
+             "FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by the factor that makes the streaming copy",
+             "kernel of the same run read and write exactly its bytes: reads x%.4f, writes x%.4f." % (fr_k, fw_k), "",
+             "| case | kernel | algorithmic MB | read MB | write MB | traffic / algorithmic |", "|---|---|---|---|---|---|"]
+    for (dim, perm, elem), i in zip(CASES, transposes[:len(CASES)]):
+        alg = 2 * int(np.prod(dim)) * elem
+        rd, wt = fetch[i] * fr_k, write[i] * fw_k
+        ent = {"dim": list(dim), "perm": list(perm), "elem": elem, "kernel": names[i],
+               "plan": plans[len(out["cases"])] if len(plans) > len(out["cases"]) else None, "algorithmic_bytes": alg,
+               "read_bytes": round(rd), "write_bytes": round(wt), "traffic_bytes": round(rd + wt),
+               "ratio": round((rd + wt) / alg, 4), "read_ratio": round(rd / (alg / 2), 4),
+               "write_ratio": round(wt / (alg / 2), 4)}
+        out["cases"].append(ent)
+        lines.append("| %s %s %dB | `%s` | %.1f | %.1f | %.1f | %.3f (reads %.3f, writes %.3f) |" % (
+            tuple(dim), tuple(perm), elem, names[i].split("(")[0], alg / 1e6, rd / 1e6, wt / 1e6, ent["ratio"],
+            ent["read_ratio"], ent["write_ratio"]))
+    print("\n".join(lines))
+    if json_path:
+        with open(json_path, "w") as fh:
+            fh.write("{\"fetch_factor\": %s, \"write_factor\": %s, \"cases\": [\n" % (out["fetch_factor"],
+                                                                                   out["write_factor"]))
+            fh.write(",\n".join(json.dumps(c) for c in out["cases"]))
+            fh.write("\n]}\n")
+
+
+def measured_traffic(dim, perm, elem):
+    """HBM bytes per launch of a named case from the committed counter summary, or None."""
+    try:
+        data = json.load(open(TRAFFIC_FILE))
+    except (OSError, ValueError):
+        return None
+    for c in data.get("cases", []):
+        if c["dim"] == list(dim) and c["perm"] == list(perm) and c["elem"] == elem:
+            return c
+    return None
 
 
 def cpu_baseline(reps):

@@ -54,13 +54,42 @@ void batch_regroup(BatchDesc& d, long long unit_bytes);
 // LIBRETT_BATCH_UNITS_PER_WG).
 long long batch_unit_bytes(long long total_bytes, int grid);
 
-// Launches the persistent kernel over `total` units of `nprob` problems.
-// `start` holds the first unit of every problem plus `total` at the end.
-hipError_t launch_batch(const BatchDesc* d_descs, const int* d_start, int nprob, int total, int grid,
+// The persistent kernel comes in one instantiation per family of shapes, so
+// that each holds only the registers its own shapes need.
+enum BatchFamily : int { kFamTiled = 0, kFamCopy = 1, kFamPacked = 2 };
+constexpr int kBatchFamilies = 3;
+
+// Family of a shape-table index.
+int batch_family(int variant);
+
+// Descriptors grouped by family: family f owns descriptors
+// [first[f], first[f] + n[f]) and start[start_off[f] ...] holds the first
+// unit of each of them and then the family's unit total.
+struct BatchLayout {
+  int n[kBatchFamilies] = {0, 0, 0};
+  int first[kBatchFamilies] = {0, 0, 0};
+  long long total[kBatchFamilies] = {0, 0, 0};
+  int start_off[kBatchFamilies] = {0, 0, 0};
+  std::vector<int> start;  // descriptors + kBatchFamilies entries
+};
+
+// Orders `descs` by family (stable) and fills the layout; order[k] is the
+// original index of the k-th ordered descriptor.
+void batch_layout(std::vector<BatchDesc>& descs, BatchLayout& layout, std::vector<int>* order);
+
+// Launches family `fam`'s persistent kernel over `total` units of `nprob`
+// descriptors (`start`: first unit of each, then `total`), with at most
+// `grid` workgroups.
+hipError_t launch_batch(int fam, const BatchDesc* d_descs, const int* d_start, int nprob, int total, int grid,
                         hipStream_t stream);
 
-// Resident workgroups per CU of the persistent kernel (runtime query).
-int batch_occupancy();
+// One launch per non-empty family, in family order on one stream; grid[f]
+// caps family f's workgroups.
+hipError_t launch_batch_all(const BatchDesc* d_descs, const int* d_start, const BatchLayout& layout, const int* grid,
+                            hipStream_t stream);
+
+// Resident workgroups per CU of family `fam`'s kernel (runtime query).
+int batch_occupancy(int fam);
 
 // Descriptors of one problem planned with candidate c: one descriptor, or
 // up to two for a plain copy (Trivial). `fallback` gets the variants for
@@ -78,5 +107,8 @@ std::vector<Candidate> batch_candidates(const Problem& p, const DeviceInfo& dev)
 // Host replay of the persistent kernel (descriptors hold host pointers).
 // Returns false if a unit wrote past the kernel's LDS.
 bool replay_batch(const BatchDesc* descs, const int* start, int nprob, int total, int grid);
+
+// Host replay of every family of a layout, `grid` workgroups each.
+bool replay_batch_all(const BatchDesc* descs, const BatchLayout& layout, int grid);
 
 }  // namespace lrt

@@ -97,16 +97,12 @@ __device__ __forceinline__ void dev_packed(const BatchDesc& D, int lu, unsigned 
   }
 }
 
-__global__ __launch_bounds__(256) void k_batch(const BatchDesc* __restrict__ descs, const int* __restrict__ start,
-                                               int nprob, int total) {
-  __shared__ __attribute__((aligned(16))) unsigned char lds[kBatchLds];
-  int u = blockIdx.x;
-  if (u >= total) return;
-  int p = batch_find(start, nprob, u);
-  for (; u < total; u += gridDim.x) {
-    while (start[p + 1] <= u) p++;
-    const BatchDesc& D = descs[p];
-    const int lu = u - start[p];
+// Runs unit lu of descriptor D if its shape belongs to family FAM (the
+// switch of each family kernel holds only that family's shapes, which keeps
+// its register use to what those shapes need).
+template <int FAM>
+__device__ __forceinline__ void dev_unit(const BatchDesc& D, int lu, unsigned char* lds) {
+  if constexpr (FAM == kFamTiled) {
     switch (D.variant) {
       case 0: dev_tiled<4, 64, 64, 1, 1>(D, lu, lds); break;
       case 1: dev_tiled<4, 64, 64, 4, 1>(D, lu, lds); break;
@@ -117,12 +113,20 @@ __global__ __launch_bounds__(256) void k_batch(const BatchDesc* __restrict__ des
       case 6: dev_tiled<8, 32, 32, 1, 2>(D, lu, lds); break;
       case 7: dev_tiled<8, 32, 32, 2, 2>(D, lu, lds); break;
       case 8: dev_tiled<16, 32, 32, 1, 1>(D, lu, lds); break;
+      default: break;
+    }
+  } else if constexpr (FAM == kFamCopy) {
+    switch (D.variant) {
       case 9: dev_copy<4, 16>(D, lu); break;
       case 10: dev_copy<4, 64>(D, lu); break;
       case 11: dev_copy<8, 16>(D, lu); break;
       case 12: dev_copy<8, 64>(D, lu); break;
       case 13: dev_copy<16, 16>(D, lu); break;
       case 14: dev_copy<16, 64>(D, lu); break;
+      default: break;
+    }
+  } else {
+    switch (D.variant) {
       case 15: dev_packed<4, 1>(D, lu, lds); break;
       case 16: dev_packed<4, 2>(D, lu, lds); break;
       case 17: dev_packed<4, 4>(D, lu, lds); break;
@@ -136,6 +140,30 @@ __global__ __launch_bounds__(256) void k_batch(const BatchDesc* __restrict__ des
       case 25: dev_packed<16, 4>(D, lu, lds); break;
       default: break;
     }
+  }
+}
+
+// Persistent kernel of one family over `total` units of `nprob`
+// descriptors: workgroup g takes units g, g + G, g + 2G, ...
+template <int FAM>
+__global__ __launch_bounds__(256) void k_batch(const BatchDesc* __restrict__ descs, const int* __restrict__ start,
+                                               int nprob, int total) {
+  // TiledCopy moves data through registers only.
+  __shared__ __attribute__((aligned(16))) unsigned char lds[FAM == kFamCopy ? 16 : kBatchLds];
+  int u = blockIdx.x;
+  if (u >= total) return;
+  int p = batch_find(start, nprob, u);
+  for (; u < total; u += gridDim.x) {
+    while (start[p + 1] <= u) p++;
+    dev_unit<FAM>(descs[p], u - start[p], lds);
+  }
+}
+
+const void* batch_kernel(int fam) {
+  switch (fam) {
+    case kFamTiled: return reinterpret_cast<const void*>(&k_batch<kFamTiled>);
+    case kFamCopy: return reinterpret_cast<const void*>(&k_batch<kFamCopy>);
+    default: return reinterpret_cast<const void*>(&k_batch<kFamPacked>);
   }
 }
 
@@ -309,21 +337,74 @@ long long batch_unit_bytes(long long total_bytes, int grid) {
   return std::max<long long>(16 << 10, std::min<long long>(kBatchUnitBytes, per));
 }
 
-hipError_t launch_batch(const BatchDesc* d_descs, const int* d_start, int nprob, int total, int grid,
+int batch_family(int variant) { return variant <= 8 ? kFamTiled : variant <= 14 ? kFamCopy : kFamPacked; }
+
+hipError_t launch_batch(int fam, const BatchDesc* d_descs, const int* d_start, int nprob, int total, int grid,
                         hipStream_t stream) {
-  if (total <= 0) return hipSuccess;
+  if (total <= 0 || nprob <= 0) return hipSuccess;
+  grid = std::max(1, std::min(grid, total));
   void* args[4] = {&d_descs, &d_start, &nprob, &total};
-  return hipLaunchKernel(reinterpret_cast<const void*>(&k_batch), dim3(grid), dim3(256), args, 0, stream);
+  return hipLaunchKernel(batch_kernel(fam), dim3(grid), dim3(256), args, 0, stream);
 }
 
-int batch_occupancy() {
+int batch_occupancy(int fam) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&k_batch), 256, 0) !=
-      hipSuccess) {
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel(fam), 256, 0) != hipSuccess) {
     (void)hipGetLastError();
     return 0;
   }
   return n;
+}
+
+void batch_layout(std::vector<BatchDesc>& descs, BatchLayout& L, std::vector<int>* order) {
+  const int nd = (int)descs.size();
+  std::vector<int> idx(nd);
+  for (int k = 0; k < nd; k++) idx[k] = k;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+    return batch_family(descs[a].variant) < batch_family(descs[b].variant);
+  });
+  std::vector<BatchDesc> sorted(nd);
+  for (int k = 0; k < nd; k++) sorted[k] = descs[idx[k]];
+  descs.swap(sorted);
+  if (order) *order = idx;
+  L.start.assign(nd + kBatchFamilies, 0);
+  int k = 0, off = 0;
+  for (int f = 0; f < kBatchFamilies; f++) {
+    L.first[f] = k;
+    L.start_off[f] = off;
+    long long tot = 0;
+    int n = 0;
+    while (k < nd && batch_family(descs[k].variant) == f) {
+      L.start[off + n] = (int)std::min<long long>(tot, 0x7fffffff);
+      tot += descs[k].units;
+      n++;
+      k++;
+    }
+    L.start[off + n] = (int)std::min<long long>(tot, 0x7fffffff);
+    L.n[f] = n;
+    L.total[f] = tot;
+    off += n + 1;
+  }
+}
+
+hipError_t launch_batch_all(const BatchDesc* d_descs, const int* d_start, const BatchLayout& L, const int* grid,
+                            hipStream_t stream) {
+  for (int f = 0; f < kBatchFamilies; f++) {
+    if (L.n[f] == 0) continue;
+    const hipError_t e =
+        launch_batch(f, d_descs + L.first[f], d_start + L.start_off[f], L.n[f], (int)L.total[f], grid[f], stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+bool replay_batch_all(const BatchDesc* descs, const BatchLayout& L, int grid) {
+  for (int f = 0; f < kBatchFamilies; f++) {
+    if (L.n[f] == 0) continue;
+    if (!replay_batch(descs + L.first[f], L.start.data() + L.start_off[f], L.n[f], (int)L.total[f], grid))
+      return false;
+  }
+  return true;
 }
 
 bool replay_batch(const BatchDesc* descs, const int* start, int nprob, int total, int grid) {

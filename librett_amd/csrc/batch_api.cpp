@@ -99,9 +99,10 @@ struct BatchObj {
   std::vector<BatchDesc> descs, fallback;
   std::vector<int> owner;               // problem of each descriptor
   std::vector<long long> byte_offset;   // offset of the descriptor's data in its problem
-  std::vector<int> start;               // first unit of each descriptor, then the total
-  int total = 0;
-  int grid = 0;
+  BatchLayout layout;                   // descriptors grouped by kernel family
+  long long total = 0;                  // units of all families
+  int grid[kBatchFamilies] = {0, 0, 0}; // workgroups of each family's launch
+  int grid_max = 0;
   long long unit_bytes = kBatchUnitBytes;
   BatchDesc* d_descs = nullptr;
   int* d_start = nullptr;
@@ -141,10 +142,12 @@ std::string problem_key(const Problem& p) {
 // Times candidate c of problem p inside the persistent kernel, on enough
 // clones of the problem to fill the device (scratch buffers `a`, `b`).
 double time_in_batch(const Candidate& c, const Problem& p, void* a, void* b, size_t scratch, hipStream_t stream,
-                     BatchDesc* d_desc, int* d_start, int max_clones, int grid, long long unit_bytes) {
+                     BatchDesc* d_desc, int* d_start, int max_clones, const int* grids, long long unit_bytes) {
   BatchDesc d;
   if (!batch_describe(c, d)) return -1;
   batch_regroup(d, unit_bytes);
+  const int fam = batch_family(d.variant);
+  const int grid = grids[fam];
   const size_t bytes = (size_t)p.vol * p.elem;
   const size_t span = (bytes + 255) / 256 * 256;
   if (span > scratch) return -1;  // every clone must lie inside the scratch buffers
@@ -168,11 +171,11 @@ double time_in_batch(const Candidate& c, const Problem& p, void* a, void* b, siz
     (void)hipEventDestroy(e0);
     return -1;
   }
-  bool ok = launch_batch(d_desc, d_start, clones, st[clones], grid, stream) == hipSuccess;
+  bool ok = launch_batch(fam, d_desc, d_start, clones, st[clones], grid, stream) == hipSuccess;
   float best = 1e30f;
   for (int r = 0; ok && r < 2; r++) {
-    ok = hipEventRecord(e0, stream) == hipSuccess && launch_batch(d_desc, d_start, clones, st[clones], grid, stream) ==
-                                                         hipSuccess &&
+    ok = hipEventRecord(e0, stream) == hipSuccess &&
+         launch_batch(fam, d_desc, d_start, clones, st[clones], grid, stream) == hipSuccess &&
          hipEventRecord(e1, stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
     float ms = 0;
     if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) best = std::min(best, ms);
@@ -199,11 +202,15 @@ int build_batch(BatchObj& B, const DeviceInfo& dev, int tune) {
   }
   B.plan_s = now_s() - t0;
 
-  const int occ = std::max(1, dev.has_gpu ? batch_occupancy() : 4);
-  B.grid = dev.cu_count * occ;
+  B.grid_max = 0;
+  for (int f = 0; f < kBatchFamilies; f++) {
+    const int occ = std::max(1, dev.has_gpu ? batch_occupancy(f) : 4);
+    B.grid[f] = dev.cu_count * occ;
+    B.grid_max = std::max(B.grid_max, B.grid[f]);
+  }
   long long total_bytes = 0;
   for (const Problem& p : B.probs) total_bytes += 2LL * p.vol * p.elem;
-  B.unit_bytes = batch_unit_bytes(total_bytes, B.grid);
+  B.unit_bytes = batch_unit_bytes(total_bytes, B.grid_max);
 
   if (tune && dev.has_gpu) {
     // Autotuner: time the top candidates of each distinct problem inside the
@@ -273,16 +280,26 @@ int build_batch(BatchObj& B, const DeviceInfo& dev, int tune) {
       B.byte_offset.push_back(offs[k]);
     }
   }
-  const int nd = (int)B.descs.size();
-  B.start.assign(nd + 1, 0);
-  long long total = 0;
-  for (int k = 0; k < nd; k++) {
-    B.start[k] = (int)total;
-    total += B.descs[k].units;
-    if (total >= (1LL << 31)) return LIBRETT_C_INVALID_PARAMETER;
+  // Group by kernel family; the fallback descriptors, owners and offsets
+  // follow the same order (a fallback keeps its descriptor's family).
+  std::vector<int> order;
+  batch_layout(B.descs, B.layout, &order);
+  std::vector<BatchDesc> fb(order.size());
+  std::vector<int> own(order.size());
+  std::vector<long long> off(order.size());
+  for (size_t k = 0; k < order.size(); k++) {
+    fb[k] = B.fallback[order[k]];
+    own[k] = B.owner[order[k]];
+    off[k] = B.byte_offset[order[k]];
   }
-  B.start[nd] = (int)total;
-  B.total = (int)total;
+  B.fallback.swap(fb);
+  B.owner.swap(own);
+  B.byte_offset.swap(off);
+  B.total = 0;
+  for (int f = 0; f < kBatchFamilies; f++) {
+    if (B.layout.total[f] >= (1LL << 31)) return LIBRETT_C_INVALID_PARAMETER;
+    B.total += B.layout.total[f];
+  }
   return LIBRETT_C_SUCCESS;
 }
 
@@ -416,11 +433,12 @@ int librett_c_batch_plan(unsigned int* handle, int count, const int* ranks, cons
     if (r != LIBRETT_C_SUCCESS) return r;
     const size_t nd = B->descs.size();
     if (hipMalloc((void**)&B->d_descs, nd * sizeof(BatchDesc)) != hipSuccess ||
-        hipMalloc((void**)&B->d_start, (nd + 1) * sizeof(int)) != hipSuccess) {
+        hipMalloc((void**)&B->d_start, B->layout.start.size() * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();
       return fail("descriptor allocation failed");
     }
-    if (hipMemcpy(B->d_start, B->start.data(), (nd + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(B->d_start, B->layout.start.data(), B->layout.start.size() * sizeof(int), hipMemcpyHostToDevice) !=
+        hipSuccess)
       return fail("descriptor upload failed");
     const unsigned int h = g_next_batch.fetch_add(1);
     {
@@ -453,20 +471,22 @@ int librett_c_batch_set_buffers(unsigned int handle, const void* const* idata, v
     up[k].out = out;
   }
   // A fallback descriptor may have another unit count: rebuild the index
-  // space and re-capture the graph when it changes.
-  std::vector<int> st(up.size() + 1, 0);
+  // space and re-capture the graph when it changes. The descriptors are
+  // already in family order, and a fallback keeps its family, so the
+  // layout keeps that order.
+  BatchLayout L;
+  std::vector<BatchDesc> tmp = up;
+  batch_layout(tmp, L, nullptr);
   long long total = 0;
-  for (size_t k = 0; k < up.size(); k++) {
-    st[k] = (int)total;
-    total += up[k].units;
+  for (int f = 0; f < kBatchFamilies; f++) {
+    if (L.total[f] >= (1LL << 31)) return LIBRETT_C_INVALID_PARAMETER;
+    total += L.total[f];
   }
-  st[up.size()] = (int)total;
-  if (total >= (1LL << 31)) return LIBRETT_C_INVALID_PARAMETER;
-  if (st != B->start) {
-    B->start = st;
-    B->total = (int)total;
-    if (hipMemcpyAsync(B->d_start, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice, B->stream) !=
-        hipSuccess)
+  if (L.start != B->layout.start) {
+    B->layout = L;
+    B->total = total;
+    if (hipMemcpyAsync(B->d_start, L.start.data(), L.start.size() * sizeof(int), hipMemcpyHostToDevice,
+                       B->stream) != hipSuccess)
       return LIBRETT_C_INTERNAL_ERROR;
     if (B->exec) {
       (void)hipGraphExecDestroy(B->exec);
@@ -491,9 +511,8 @@ int librett_c_batch_execute(unsigned int handle, int use_graph) {
   auto B = find_batch(handle);
   if (!B) return LIBRETT_C_INVALID_PLAN;
   if (!B->buffers_set) return LIBRETT_C_INVALID_PARAMETER;
-  const int nd = (int)B->descs.size();
   if (!use_graph) {
-    return launch_batch(B->d_descs, B->d_start, nd, B->total, B->grid, B->stream) == hipSuccess
+    return launch_batch_all(B->d_descs, B->d_start, B->layout, B->grid, B->stream) == hipSuccess
                ? LIBRETT_C_SUCCESS
                : LIBRETT_C_INTERNAL_ERROR;
   }
@@ -502,7 +521,7 @@ int librett_c_batch_execute(unsigned int handle, int use_graph) {
       (void)hipGetLastError();
       return LIBRETT_C_INTERNAL_ERROR;
     }
-    const hipError_t le = launch_batch(B->d_descs, B->d_start, nd, B->total, B->grid, B->stream);
+    const hipError_t le = launch_batch_all(B->d_descs, B->d_start, B->layout, B->grid, B->stream);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(B->stream, &g);
     if (le != hipSuccess || ce != hipSuccess || !g) {
@@ -537,8 +556,8 @@ int librett_c_batch_info_get(unsigned int handle, librett_c_batch_info* info) {
   std::memset(info, 0, sizeof(*info));
   info->count = B->nprob;
   info->descriptors = (int)B->descs.size();
-  info->total_units = B->total;
-  info->grid = B->grid;
+  info->total_units = (int)std::min<long long>(B->total, 0x7fffffff);
+  info->grid = B->grid_max;
   info->tuned = B->tuned;
   info->plan_seconds = B->plan_s;
   info->tune_seconds = B->tune_s;
@@ -583,8 +602,7 @@ int librett_c_batch_host_replay(int count, const int* ranks, const int* dims, co
       ds[k].in = gin[i].data() + g + B.byte_offset[k];
       ds[k].out = gout[i].data() + g + B.byte_offset[k];
     }
-    if (!replay_batch(ds.data(), B.start.data(), (int)ds.size(), B.total, grid > 0 ? grid : B.grid))
-      return LIBRETT_C_UNDEFINED_ERROR;
+    if (!replay_batch_all(ds.data(), B.layout, grid > 0 ? grid : B.grid_max)) return LIBRETT_C_UNDEFINED_ERROR;
     for (int i = 0; i < count; i++) {
       const size_t n = (size_t)B.probs[i].vol * B.probs[i].elem;
       for (size_t j = 0; j < g; j++)

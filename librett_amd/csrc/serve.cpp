@@ -115,6 +115,11 @@ class Arena {
     std::lock_guard<std::mutex> lock(mu_);
     return peak_;
   }
+  // Restarts the peak at the bytes in use now.
+  void reset_peak() {
+    std::lock_guard<std::mutex> lock(mu_);
+    peak_ = in_use_;
+  }
   size_t capacity() const { return cap_; }
   bool owns(const void* p) const { return p >= base_ && p < (const char*)base_ + cap_; }
 
@@ -166,7 +171,11 @@ class Server {
     dev_.max_waves_per_cu = std::max(1, prop.maxThreadsPerMultiProcessor / std::max(1, prop.warpSize));
     if (prop.l2CacheSize > 0) dev_.l2_bytes = (double)prop.l2CacheSize * std::max(1, dev_.cu_count / 32);
     dev_.has_gpu = 1;
-    grid_ = dev_.cu_count * std::max(1, batch_occupancy());
+    grid_max_ = 0;
+    for (int f = 0; f < kBatchFamilies; f++) {
+      grid_[f] = dev_.cu_count * std::max(1, batch_occupancy(f));
+      grid_max_ = std::max(grid_max_, grid_[f]);
+    }
     max_batch_ = std::max(1, max_batch);
     if (!arena_.init(arena_bytes)) return LIBRETT_C_INTERNAL_ERROR;
     slots_.resize(std::max(1, nslots));
@@ -175,9 +184,10 @@ class Server {
       if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
           hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
           hipHostMalloc((void**)&s.h_desc, s.cap * sizeof(BatchDesc), hipHostMallocDefault) != hipSuccess ||
-          hipHostMalloc((void**)&s.h_start, (s.cap + 1) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+          hipHostMalloc((void**)&s.h_start, (s.cap + kBatchFamilies) * sizeof(int), hipHostMallocDefault) !=
+              hipSuccess ||
           hipMalloc((void**)&s.d_desc, s.cap * sizeof(BatchDesc)) != hipSuccess ||
-          hipMalloc((void**)&s.d_start, (s.cap + 1) * sizeof(int)) != hipSuccess) {
+          hipMalloc((void**)&s.d_start, (s.cap + kBatchFamilies) * sizeof(int)) != hipSuccess) {
         (void)hipGetLastError();
         return LIBRETT_C_INTERNAL_ERROR;
       }
@@ -302,6 +312,7 @@ class Server {
     first_arrival_ = 1e300;
     last_done_ = 0;
     finished_.clear();
+    arena_.reset_peak();
   }
 
   void add_stall(double s) {
@@ -363,21 +374,21 @@ class Server {
     if (take.empty()) return false;
     long long bytes = 0;
     for (auto& r : take) bytes += 2LL * r->prob.vol * r->prob.elem;
-    const long long unit_bytes = batch_unit_bytes(bytes, grid_);
+    const long long unit_bytes = batch_unit_bytes(bytes, grid_max_);
     for (int k = 0; k < ndesc; k++) batch_regroup(use[k], unit_bytes);
-    int total = 0;
-    for (int k = 0; k < ndesc; k++) {
-      s.h_desc[k] = use[k];
-      s.h_start[k] = total;
-      total += use[k].units;
-    }
-    s.h_start[ndesc] = total;
-    const int grid = std::max(1, std::min(grid_, total));
-    bool ok = hipMemcpyAsync(s.d_desc, s.h_desc, ndesc * sizeof(BatchDesc), hipMemcpyHostToDevice, s.stream) ==
+    // One launch per kernel family, in family order on the slot's stream.
+    BatchLayout L;
+    batch_layout(use, L, nullptr);
+    bool fits = true;
+    for (int f = 0; f < kBatchFamilies; f++) fits = fits && L.total[f] < (1LL << 31);
+    for (int k = 0; k < ndesc; k++) s.h_desc[k] = use[k];
+    for (size_t k = 0; k < L.start.size(); k++) s.h_start[k] = L.start[k];
+    bool ok = fits &&
+              hipMemcpyAsync(s.d_desc, s.h_desc, ndesc * sizeof(BatchDesc), hipMemcpyHostToDevice, s.stream) ==
                   hipSuccess &&
-              hipMemcpyAsync(s.d_start, s.h_start, (ndesc + 1) * sizeof(int), hipMemcpyHostToDevice, s.stream) ==
+              hipMemcpyAsync(s.d_start, s.h_start, L.start.size() * sizeof(int), hipMemcpyHostToDevice, s.stream) ==
                   hipSuccess &&
-              launch_batch(s.d_desc, s.d_start, ndesc, total, grid, s.stream) == hipSuccess &&
+              launch_batch_all(s.d_desc, s.d_start, L, grid_, s.stream) == hipSuccess &&
               hipEventRecord(s.done, s.stream) == hipSuccess;
     if (!ok) {
       (void)hipGetLastError();
@@ -456,7 +467,8 @@ class Server {
 
   int device_ = 0;
   DeviceInfo dev_;
-  int grid_ = 0;
+  int grid_[kBatchFamilies] = {0, 0, 0};
+  int grid_max_ = 0;
   int max_batch_ = 64;
   Arena arena_;
   std::vector<Slot> slots_;
