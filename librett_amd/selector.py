@@ -13,9 +13,13 @@ to the analytic model when no weights are loaded.
 
 Command line (on a GPU)::
 
-    python -m librett_amd.selector collect --problems 20000 --seed 1 --out train.npz
-    python -m librett_amd.selector train --data train.npz --weights librett_amd/selector_weights.bin
-    python -m librett_amd.selector evaluate --problems 2000 --seed 2 --weights librett_amd/selector_weights.bin
+    # timings of problems [0, 10000) and [10000, 20000) of seed 1 (two runs)
+    python -m librett_amd.selector collect --problems 10000 --total 20000 --offset 0 --seed 1 --out a.npz
+    python -m librett_amd.selector collect --problems 10000 --total 20000 --offset 10000 --seed 1 --out b.npz
+    # features recomputed on this GPU, training, held-out evaluation (seed + 1000)
+    python -m librett_amd.selector train --data a.npz b.npz --heldout 2000 --seed 1 --weights w.bin --report r.json
+    python -m librett_amd.selector evaluate --problems 2000 --seed 1001 --weights librett_amd/selector_weights.bin
+    # all in one run
     python -m librett_amd.selector build --train 20000 --heldout 2000 --weights ... --report report.json
 
 The reference has no learned selector; its heuristic is choosePlanHeuristic
@@ -125,29 +129,54 @@ def collect(probs, stream, in_ptr, out_ptr, log=None):
     return recs
 
 
-def save_records(path, recs):
+def save_records(path, recs, with_features=True):
+    """Records to a compressed .npz; without features it holds only the
+    problems and the measured times (features are recomputed with
+    attach_features on the same GPU model)."""
     off = np.cumsum([0] + [len(r["times"]) for r in recs])
-    np.savez_compressed(
-        path,
-        feats=np.concatenate([r["feats"] for r in recs]).astype(np.float32),
-        times=np.concatenate([r["times"] for r in recs]),
+    arrays = dict(
+        times=np.concatenate([r["times"] for r in recs]).astype(np.float32),
         offsets=off,
         model_choice=np.array([r["model_choice"] for r in recs]),
         elem=np.array([r["elem"] for r in recs]),
         problems=np.array([json.dumps([r["dims"], r["perm"]]) for r in recs]),
     )
+    if with_features:
+        arrays["feats"] = np.concatenate([r["feats"] for r in recs]).astype(np.float32)
+    np.savez_compressed(path, **arrays)
 
 
 def load_records(path):
     z = np.load(path, allow_pickle=False)
     off = z["offsets"]
+    has_f = "feats" in z.files
     recs = []
     for i in range(len(off) - 1):
         a, b = off[i], off[i + 1]
         d, p = json.loads(str(z["problems"][i]))
-        recs.append({"dims": d, "perm": p, "elem": int(z["elem"][i]), "feats": z["feats"][a:b],
-                     "times": z["times"][a:b], "model_choice": int(z["model_choice"][i])})
+        recs.append({"dims": d, "perm": p, "elem": int(z["elem"][i]), "feats": z["feats"][a:b] if has_f else None,
+                     "times": z["times"][a:b].astype(np.float64), "model_choice": int(z["model_choice"][i])})
     return recs
+
+
+def attach_features(recs, log=None):
+    """Recomputes every record's candidate features on the current GPU;
+    drops records whose candidate list no longer matches their times."""
+    from . import selector_features
+    out = []
+    t0 = last = time.time()
+    for k, r in enumerate(recs):
+        feats, mc = selector_features(r["dims"], r["perm"], r["elem"])
+        if len(feats) == len(r["times"]) and mc == r["model_choice"]:
+            r = dict(r)
+            r["feats"] = feats
+            out.append(r)
+        if log and time.time() - last > 20:
+            last = time.time()
+            log("features %d/%d (%.0f s)" % (k + 1, len(recs), last - t0))
+    if log:
+        log("features for %d of %d records" % (len(out), len(recs)))
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -268,11 +297,13 @@ def main(argv=None):
     ap.add_argument("cmd", choices=["collect", "train", "evaluate", "build"])
     ap.add_argument("--train", type=int, default=20000, help="build: training problems (seed --seed)")
     ap.add_argument("--heldout", type=int, default=2000, help="build: held-out problems (seed --seed + 1000)")
-    ap.add_argument("--report", default=None, help="build: JSON report path")
+    ap.add_argument("--report", default=None, help="build / train: JSON report path")
+    ap.add_argument("--total", type=int, default=None, help="collect: size of the seeded problem list")
+    ap.add_argument("--offset", type=int, default=0, help="collect: first problem of the list to time")
     ap.add_argument("--problems", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--out", default="selector_data.npz")
-    ap.add_argument("--data", default=None)
+    ap.add_argument("--data", nargs="+", default=None)
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -307,18 +338,39 @@ def main(argv=None):
             save_records(a.out, ho)
     elif a.cmd == "collect":
         A, B = _buffers(512 << 20)
-        recs = collect(problems(a.problems, a.seed), stream, A.ptr, B.ptr, log)
-        save_records(a.out, recs)
+        probs = problems(a.total or a.problems, a.seed)[a.offset:a.offset + a.problems]
+        recs = collect(probs, stream, A.ptr, B.ptr, log)
+        save_records(a.out, recs, with_features=False)
     elif a.cmd == "train":
-        recs = load_records(a.data)
+        recs = []
+        for path in a.data:
+            recs += load_records(path)
+        if any(r["feats"] is None for r in recs):
+            recs = attach_features(recs, log)
+        t1 = time.time()
         p, mean, std, losses = train(recs, stream, a.epochs, a.lr, log=log)
+        t2 = time.time()
         save_weights(a.weights, p, mean, std)
-        sel, ana = evaluate(recs, scores(p, mean, std, recs, stream))
-        print(json.dumps({"train": {"selector": summary(sel), "analytic": summary(ana),
-                                    "final_mse": float(losses[-1])}}))
+        A, B = _buffers(512 << 20)
+        ho = collect(problems(a.heldout, a.seed + 1000), stream, A.ptr, B.ptr, log)
+        s_tr, a_tr = evaluate(recs, scores(p, mean, std, recs, stream))
+        s_ho, a_ho = evaluate(ho, scores(p, mean, std, ho, stream))
+        rep = {"train_problems": len(recs), "heldout_problems": len(ho),
+               "samples": int(sum(len(r["times"]) for r in recs)), "epochs": a.epochs, "lr": a.lr,
+               "final_mse": float(losses[-1]), "train_s": t2 - t1, "seed": a.seed,
+               "train": {"selector": summary(s_tr), "analytic": summary(a_tr)},
+               "heldout": {"selector": summary(s_ho), "analytic": summary(a_ho)}}
+        print(json.dumps(rep))
+        if a.report:
+            with open(a.report, "w") as fh:
+                json.dump(rep, fh, indent=1)
     else:
         if a.data:
-            recs = load_records(a.data)
+            recs = []
+            for path in a.data:
+                recs += load_records(path)
+            if any(r["feats"] is None for r in recs):
+                recs = attach_features(recs, log)
         else:
             A, B = _buffers(512 << 20)
             recs = collect(problems(a.problems, a.seed), stream, A.ptr, B.ptr, log)

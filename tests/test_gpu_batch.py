@@ -152,3 +152,36 @@ def test_batch_autotuned_problem_over_256mib(gpu_stream):
     assert not slab.check()
     b.destroy()
     slab.free()
+
+
+def test_autotuner_not_slower_than_model(gpu_stream):
+    # The autotuner times the model's best candidates (the model's own choice
+    # among them) inside the persistent kernel, so its batch may not be
+    # slower than the model's batch beyond timing noise (5%).
+    probs = make_problems(400, 9, vol_hi=200000)
+    say("case: autotuned vs model batch timing,", len(probs), "problems")
+    slab = Slab(probs, seed=3)
+    times = {}
+    for tune in (0, 3):
+        b = L.Batch(probs, gpu_stream, tune=tune)
+        b.set_buffers(slab.in_ptrs(), slab.out_ptrs())
+        e0, e1 = hip.Event(), hip.Event()
+        ts = []
+        for _ in range(3):
+            b.execute(use_graph=True)
+        for _ in range(7):
+            e0.record(gpu_stream)
+            for _ in range(5):
+                b.execute(use_graph=True)
+            e1.record(gpu_stream)
+            e1.synchronize()
+            ts.append(e1.elapsed_ms(e0) / 5)
+        times[tune] = sorted(ts)[len(ts) // 2]
+        hip.memset(slab.b.ptr, 0xEE, slab.total)
+        assert b.execute(use_graph=True) == L.LIBRETT_SUCCESS
+        hip.stream_sync(gpu_stream)
+        assert not slab.check()
+        b.destroy()
+    say("  model %.3f ms, autotuned %.3f ms" % (times[0], times[3]))
+    assert times[3] <= 1.05 * times[0]
+    slab.free()

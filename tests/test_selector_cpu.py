@@ -149,3 +149,21 @@ def test_problem_generator_ranges():
     for d, p, e in S.problems(300, 5):
         assert 2 <= len(d) <= 7 and sorted(p) == list(range(len(d))) and e in (4, 8, 16)
         assert np.prod(d) * e <= 512 << 20 and min(d) >= 2
+
+
+def test_records_roundtrip_without_features(tmp_path):
+    recs = [{"dims": [4, 5, 6], "perm": [2, 0, 1], "elem": 8, "feats": np.ones((3, M.F), np.float32),
+             "times": np.array([1e-3, -1.0, 2e-3]), "model_choice": 2},
+            {"dims": [7, 9], "perm": [1, 0], "elem": 4, "feats": np.zeros((1, M.F), np.float32),
+             "times": np.array([5e-4]), "model_choice": 0}]
+    path = str(tmp_path / "r.npz")
+    S.save_records(path, recs, with_features=False)
+    back = S.load_records(path)
+    assert len(back) == 2 and back[0]["feats"] is None
+    for r, b in zip(recs, back):
+        assert r["dims"] == b["dims"] and r["perm"] == b["perm"] and r["elem"] == b["elem"]
+        assert r["model_choice"] == b["model_choice"]
+        assert np.allclose(r["times"], b["times"], rtol=1e-6)
+    # Candidates that did not run stay marked, and the targets skip them.
+    y = S.targets(back[0])
+    assert np.isnan(y[1]) and np.all(np.isfinite(y[[0, 2]]))
