@@ -312,6 +312,24 @@ int librett_c_mlp_host_grad(const float* params, const float* x, const float* y,
 void librett_c_mlp_host_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr,
                              float beta1, float beta2, float c1, float c2, float eps);
 
+/* ------------------------------------------------------------------------
+ * Index-pattern checker (extension; the reference's TensorTester,
+ * src/TensorTester.cpp:34-126,201-220). check_fill sets 32-bit word k of
+ * `data` to k (mod 2^32), so input element e of B bytes holds the words
+ * e*B/4 + w. check_transpose verifies on the GPU that `odata` is the
+ * transpose of such an input (column-major dims, output dims
+ * dim[permutation[i]]), for element sizes 4, 8 and 16; it stores the number
+ * of wrong elements and the first wrong output position (-1 if none) and
+ * synchronises `stream`. The host_ variants run the same per-element code on
+ * host memory.
+ * ------------------------------------------------------------------------ */
+int librett_c_check_fill(void* data, size_t nbytes, void* stream);
+int librett_c_check_transpose(const void* odata, int rank, const int* dim, const int* permutation, size_t sizeofType,
+                              void* stream, unsigned long long* mismatches, long long* first);
+void librett_c_host_check_fill(void* data, size_t nbytes);
+int librett_c_host_check_transpose(const void* odata, int rank, const int* dim, const int* permutation,
+                                   size_t sizeofType, unsigned long long* mismatches, long long* first);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,7 @@ EXPORTED = [
     "librett_c_selector_set", "librett_c_selector_load", "librett_c_selector_generation", "librett_c_mlp_forward",
     "librett_c_mlp_grad", "librett_c_mlp_adam", "librett_c_mlp_train", "librett_c_mlp_host_forward",
     "librett_c_mlp_host_grad", "librett_c_mlp_host_adam",
+    "librett_c_check_fill", "librett_c_check_transpose", "librett_c_host_check_fill", "librett_c_host_check_transpose",
 ]
 LIBRETT_BUSY = 6
 
@@ -191,13 +192,19 @@ def lib():
         L.librett_c_mlp_host_forward.argtypes = [fp, fp, ctypes.c_int, fp]
         L.librett_c_mlp_host_grad.argtypes = [fp, fp, fp, ip, ctypes.c_int, f_, fp, fp]
         L.librett_c_mlp_host_adam.argtypes = [fp, ctypes.c_int, fp, fp, fp, f_, f_, f_, f_, f_, f_]
+        llp = ctypes.POINTER(ctypes.c_longlong)
+        L.librett_c_check_fill.argtypes = [vp, ctypes.c_size_t, vp]
+        L.librett_c_check_transpose.argtypes = [vp, ctypes.c_int, ip, ip, ctypes.c_size_t, vp, ullp, llp]
+        L.librett_c_host_check_fill.argtypes = [vp, ctypes.c_size_t]
+        L.librett_c_host_check_transpose.argtypes = [vp, ctypes.c_int, ip, ip, ctypes.c_size_t, ullp, llp]
         L.librett_c_version.restype = ctypes.c_char_p
         for name in EXPORTED:
             f = getattr(L, name)
             if name != "librett_c_version" and f.restype is not ctypes.c_char_p:
                 f.restype = ctypes.c_int
         L.librett_c_set_model_constants.restype = None
-        for name in ("librett_c_mlp_sizes", "librett_c_mlp_host_forward", "librett_c_mlp_host_adam"):
+        for name in ("librett_c_mlp_sizes", "librett_c_mlp_host_forward", "librett_c_mlp_host_adam",
+                     "librett_c_host_check_fill"):
             getattr(L, name).restype = None
         L.librett_c_selector_generation.restype = ctypes.c_ulonglong
         L.librett_c_initialize.restype = None
@@ -604,6 +611,39 @@ def mlp_train(d_x, d_y, n, d_params, epochs, lr, minibatch, seed, stream):
     _ok(lib().librett_c_mlp_train(d_x, d_y, n, d_params, epochs, lr, minibatch, seed, _fptr(loss),
                                   ctypes.c_void_p(stream)), "mlp_train")
     return loss[:epochs]
+
+
+# ---------------------------------------------------------------------------
+# Index-pattern checker (extension; counterpart of the reference's TensorTester)
+# ---------------------------------------------------------------------------
+
+def check_fill(ptr, nbytes, stream):
+    """Device buffer: 32-bit word k := k (asynchronous on `stream`)."""
+    _ok(lib().librett_c_check_fill(ctypes.c_void_p(ptr), int(nbytes), ctypes.c_void_p(stream)), "check_fill")
+
+
+def check_transpose(optr, dim, permutation, sizeofType, stream):
+    """(wrong elements, first wrong output position or -1) of a device output."""
+    n, f = ctypes.c_ulonglong(0), ctypes.c_longlong(-1)
+    r = lib().librett_c_check_transpose(ctypes.c_void_p(optr), len(dim), _iarr(dim), _iarr(permutation),
+                                        int(sizeofType), ctypes.c_void_p(stream), ctypes.byref(n), ctypes.byref(f))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("check_transpose failed: %d" % r)
+    return n.value, f.value
+
+
+def host_check_fill(arr):
+    """numpy array (any dtype, size a multiple of 4 bytes): word k := k."""
+    lib().librett_c_host_check_fill(ctypes.c_void_p(arr.ctypes.data), arr.nbytes)
+
+
+def host_check_transpose(arr, dim, permutation, sizeofType):
+    n, f = ctypes.c_ulonglong(0), ctypes.c_longlong(-1)
+    r = lib().librett_c_host_check_transpose(ctypes.c_void_p(arr.ctypes.data), len(dim), _iarr(dim),
+                                             _iarr(permutation), int(sizeofType), ctypes.byref(n), ctypes.byref(f))
+    if r != LIBRETT_SUCCESS:
+        raise ValueError("host_check_transpose failed: %d" % r)
+    return n.value, f.value
 
 
 def version():
