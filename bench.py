@@ -53,10 +53,13 @@ def main():
                     help="librettPlan with the analytic model only (no learned selector)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=5)
-    ap.add_argument("--workload", default="large", choices=["large", "batched", "serving", "selector"],
-                    help="large: the seven ~256 MiB cases (default); batched: 4096 uneven problems; "
-                         "serving: Poisson request stream through the continuous-batching engine; "
-                         "selector: learned plan selector vs analytic model on held-out problems")
+    ap.add_argument("--workload", default="large",
+                    choices=["large", "large-random", "batched", "serving", "selector"],
+                    help="large: the seven ~256 MiB cases (default); large-random: 48 seeded ~256 MiB cases "
+                         "(keep/move/small leading dim x f32/f64 x 8 seeds), librettPlan vs librettPlanMeasure, "
+                         "checked on the device; batched: 4096 uneven problems; serving: Poisson request stream "
+                         "through the continuous-batching engine; selector: learned plan selector vs analytic "
+                         "model on held-out problems")
     ap.add_argument("--heldout", type=int, default=2000, help="selector workload: held-out problems")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "RUN_LOG"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
@@ -86,6 +89,8 @@ def main():
         return bench_serving(args, stream)
     if args.workload == "selector":
         return bench_selector(args, stream)
+    if args.workload == "large-random":
+        return bench_large_random(args, stream)
     if args.analytic:
         L.selector_set(None, None, None)
     selection = ("librettPlanMeasure" if args.measure else
@@ -254,6 +259,87 @@ def main():
         "cpu_baseline": cpu,
         "cases": per_case,
     }
+    print(json.dumps(line), flush=True)
+
+
+def bench_large_random(args, stream):
+    """Seeded ~256 MiB transposes of three classes (leading dim kept, moved,
+    small) in f32 and f64: GB/s of librettPlan's and librettPlanMeasure's
+    plans (HIP events, median of the timed steps), their ratio, the same-
+    buffer D2D copy, and an on-device check of every output."""
+    import librett_amd as L
+    from librett_amd import hip, workloads as W
+    cap = 300 << 20
+    A, B = hip.DeviceBuffer(cap), hip.DeviceBuffer(cap)
+    e0, e1 = hip.Event(), hip.Event()
+
+    def timed(fn, n):
+        fn()
+        hip.stream_sync(stream)
+        ts = []
+        for _ in range(n):
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            ts.append(e1.elapsed_ms(e0) * 1e-3)
+        return sorted(ts)[len(ts) // 2]
+
+    rows = []
+    steps = max(3, args.steps // 2)
+    for elem in (4, 8):
+        for cls in ("keep", "move", "small"):
+            for seed in range(8):
+                d, p = W.large_case(20261202 + seed, elem, cls)
+                n = int(np.prod(d)) * elem
+                L.check_fill(A.ptr, n, stream)
+                r1, h1 = L.librettPlan(len(d), d, p, elem, stream)
+                r2, h2 = L.librettPlanMeasure(len(d), d, p, elem, stream, A.ptr, B.ptr)
+                if r1 != L.LIBRETT_SUCCESS or r2 != L.LIBRETT_SUCCESS:
+                    raise SystemExit("plan failed for %s %s" % (d, p))
+                t_plan = timed(lambda: L.librettExecute(h1, A.ptr, B.ptr), steps)
+                bad1 = L.check_transpose(B.ptr, d, p, elem, stream)[0]
+                t_meas = timed(lambda: L.librettExecute(h2, A.ptr, B.ptr), steps)
+                bad2 = L.check_transpose(B.ptr, d, p, elem, stream)[0]
+                t_d2d = timed(lambda: hip.memcpy_d2d_async(B.ptr, A.ptr, n, stream), steps)
+                if bad1 or bad2:
+                    raise SystemExit("wrong output for %s %s: %d / %d elements" % (d, p, bad1, bad2))
+                row = dict(cls=cls, elem=elem, seed=seed, dim=d, perm=p, bytes=2 * n,
+                           plan=L.plan_info(h1)["desc"], measured=L.plan_info(h2)["desc"],
+                           plan_gbs=round(2 * n / t_plan / 1e9, 1), measure_gbs=round(2 * n / t_meas / 1e9, 1),
+                           d2d_gbs=round(2 * n / t_d2d / 1e9, 1), plan_over_measure=round(t_plan / t_meas, 3))
+                rows.append(row)
+                log("%-5s f%d seed %d %-32s %-22s plan %7.1f GB/s measure %7.1f GB/s (x%.3f) D2D %7.1f" % (
+                    cls, 8 * elem, seed, d, p, row["plan_gbs"], row["measure_gbs"], row["plan_over_measure"],
+                    row["d2d_gbs"]))
+                L.librettDestroy(h1)
+                L.librettDestroy(h2)
+    tot = sum(r["bytes"] for r in rows)
+    t_plan = sum(r["bytes"] / (r["plan_gbs"] * 1e9) for r in rows)
+    t_meas = sum(r["bytes"] / (r["measure_gbs"] * 1e9) for r in rows)
+    t_d2d = sum(r["bytes"] / (r["d2d_gbs"] * 1e9) for r in rows)
+    by_cls = {}
+    for cls in ("keep", "move", "small"):
+        rs = [r for r in rows if r["cls"] == cls]
+        by_cls[cls] = {"plan_gbs": round(sum(r["bytes"] for r in rs) / sum(r["bytes"] / (r["plan_gbs"] * 1e9)
+                                                                          for r in rs) / 1e9, 1),
+                       "frac_d2d_min": round(min(r["plan_gbs"] / r["d2d_gbs"] for r in rs), 3),
+                       "frac_d2d_median": round(float(np.median([r["plan_gbs"] / r["d2d_gbs"] for r in rs])), 3)}
+    line = {
+        "metric": "transpose GB/s (2*bytes/t), seeded ~256 MiB cases, librettPlan vs librettPlanMeasure",
+        "value": round(tot / t_plan / 1e9, 1), "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": round(t_plan * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64 (opaque 4/8-byte moves)", "data": "synthetic (index pattern, checked on the device)",
+        "config": {"workload": "48 seeded ~256 MiB transposes: leading dim kept / moved / small, f32 and f64, "
+                               "8 seeds each, rank 2-6"},
+        "measure_gbs": round(tot / t_meas / 1e9, 1), "d2d_gbs": round(tot / t_d2d / 1e9, 1),
+        "frac_of_d2d": round(t_d2d / t_plan, 3), "plan_over_measure_time_median":
+            float(np.median([r["plan_over_measure"] for r in rows])),
+        "plan_over_measure_time_max": max(r["plan_over_measure"] for r in rows),
+        "by_class": by_cls, "cases": rows, "roofline": None, "cpu_baseline": None,
+    }
+    A.free()
+    B.free()
     print(json.dumps(line), flush=True)
 
 
