@@ -330,6 +330,22 @@ void librett_c_host_check_fill(void* data, size_t nbytes);
 int librett_c_host_check_transpose(const void* odata, int rank, const int* dim, const int* permutation,
                                    size_t sizeofType, unsigned long long* mismatches, long long* first);
 
+/* ------------------------------------------------------------------------
+ * Device allocator hook (extension; the reference uses an Umpire allocator
+ * chosen at librettInitialize, src/librett.cpp:387-395). The library's own
+ * device buffers -- batch descriptors and autotuner scratch, the serving
+ * arena and slot buffers, selector training and checker buffers -- come
+ * from `alloc` and go back through `free_fn` when installed (both NULL:
+ * hipMalloc / hipFree). A buffer always returns to the allocator that made
+ * it. Plans own no device memory. The selector's small per-device inference
+ * buffers, kept for the life of the process, stay on hipMalloc.
+ * allocator_live counts buffers currently held from a caller's allocator.
+ * ------------------------------------------------------------------------ */
+typedef void* (*librett_c_alloc_fn)(size_t bytes, void* ctx);
+typedef void (*librett_c_free_fn)(void* ptr, void* ctx);
+int librett_c_set_allocator(librett_c_alloc_fn alloc, librett_c_free_fn free_fn, void* ctx);
+unsigned long long librett_c_allocator_live(void);
+
 #ifdef __cplusplus
 }
 #endif

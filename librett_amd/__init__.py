@@ -81,6 +81,7 @@ EXPORTED = [
     "librett_c_mlp_grad", "librett_c_mlp_adam", "librett_c_mlp_train", "librett_c_mlp_host_forward",
     "librett_c_mlp_host_grad", "librett_c_mlp_host_adam",
     "librett_c_check_fill", "librett_c_check_transpose", "librett_c_host_check_fill", "librett_c_host_check_transpose",
+    "librett_c_set_allocator", "librett_c_allocator_live",
 ]
 LIBRETT_BUSY = 6
 
@@ -119,6 +120,32 @@ class BatchInfo(ctypes.Structure):
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["method_count"] = {METHOD_NAMES.get(i, str(i)): self.method_count[i] for i in range(6) if self.method_count[i]}
         return d
+
+
+ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
+FREE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p)
+_ALLOCATOR = None  # keeps the installed callbacks alive
+
+
+def set_allocator(alloc=None, free=None):
+    """Installs Python callables alloc(nbytes) -> device pointer (int, 0 on
+    failure) and free(ptr) as the library's device allocator; no arguments
+    restore hipMalloc / hipFree."""
+    global _ALLOCATOR
+    if alloc is None:
+        r = lib().librett_c_set_allocator(ALLOC_FN(), FREE_FN(), None)
+        _ALLOCATOR = None
+        return r
+    fa = ALLOC_FN(lambda n, ctx: alloc(n) or None)
+    ff = FREE_FN(lambda p, ctx: free(p))
+    r = lib().librett_c_set_allocator(fa, ff, None)
+    if r == LIBRETT_SUCCESS:
+        _ALLOCATOR = (fa, ff)
+    return r
+
+
+def allocator_live():
+    return lib().librett_c_allocator_live()
 
 
 def lib():
@@ -197,6 +224,8 @@ def lib():
         L.librett_c_check_transpose.argtypes = [vp, ctypes.c_int, ip, ip, ctypes.c_size_t, vp, ullp, llp]
         L.librett_c_host_check_fill.argtypes = [vp, ctypes.c_size_t]
         L.librett_c_host_check_transpose.argtypes = [vp, ctypes.c_int, ip, ip, ctypes.c_size_t, ullp, llp]
+        L.librett_c_set_allocator.argtypes = [ALLOC_FN, FREE_FN, vp]
+        L.librett_c_allocator_live.argtypes = []
         L.librett_c_version.restype = ctypes.c_char_p
         for name in EXPORTED:
             f = getattr(L, name)
@@ -207,6 +236,7 @@ def lib():
                      "librett_c_host_check_fill"):
             getattr(L, name).restype = None
         L.librett_c_selector_generation.restype = ctypes.c_ulonglong
+        L.librett_c_allocator_live.restype = ctypes.c_ulonglong
         L.librett_c_initialize.restype = None
         L.librett_c_finalize.restype = None
         _LIB = L

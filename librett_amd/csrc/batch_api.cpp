@@ -16,6 +16,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "devmem.h"
 #include "../../include/librett_c.h"
 #include "batch.h"
 #include "kernels.h"
@@ -115,8 +116,8 @@ struct BatchObj {
   ~BatchObj() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (d_descs) (void)hipFree(d_descs);
-    if (d_start) (void)hipFree(d_start);
+    if (d_descs) (void)dev_free(d_descs);
+    if (d_start) (void)dev_free(d_start);
   }
 };
 
@@ -226,9 +227,9 @@ int build_batch(BatchObj& B, const DeviceInfo& dev, int tune) {
     BatchDesc* d_desc = nullptr;
     int* d_start = nullptr;
     const int max_clones = 4096;
-    bool ok = hipMalloc(&a, scratch) == hipSuccess && hipMalloc(&b, scratch) == hipSuccess &&
-              hipMalloc((void**)&d_desc, max_clones * sizeof(BatchDesc)) == hipSuccess &&
-              hipMalloc((void**)&d_start, (max_clones + 1) * sizeof(int)) == hipSuccess;
+    bool ok = dev_malloc(&a, scratch) == hipSuccess && dev_malloc(&b, scratch) == hipSuccess &&
+              dev_malloc((void**)&d_desc, max_clones * sizeof(BatchDesc)) == hipSuccess &&
+              dev_malloc((void**)&d_start, (max_clones + 1) * sizeof(int)) == hipSuccess;
     if (ok) ok = hipMemsetAsync(a, 0, scratch, B.stream) == hipSuccess;
     const int topk = std::max(1, std::min(8, tune));
     for (int i = 0; ok && i < B.nprob; i++) {
@@ -254,10 +255,10 @@ int build_batch(BatchObj& B, const DeviceInfo& dev, int tune) {
       g_tune_cache.put(key, TuneValue{B.chosen[i], best});
     }
     (void)hipStreamSynchronize(B.stream);
-    if (a) (void)hipFree(a);
-    if (b) (void)hipFree(b);
-    if (d_desc) (void)hipFree(d_desc);
-    if (d_start) (void)hipFree(d_start);
+    if (a) (void)dev_free(a);
+    if (b) (void)dev_free(b);
+    if (d_desc) (void)dev_free(d_desc);
+    if (d_start) (void)dev_free(d_start);
     if (!ok) return fail("autotuner HIP call failed");
     B.tune_s = now_s() - t1;
   }
@@ -432,8 +433,8 @@ int librett_c_batch_plan(unsigned int* handle, int count, const int* ranks, cons
     r = build_batch(*B, dev, tune);
     if (r != LIBRETT_C_SUCCESS) return r;
     const size_t nd = B->descs.size();
-    if (hipMalloc((void**)&B->d_descs, nd * sizeof(BatchDesc)) != hipSuccess ||
-        hipMalloc((void**)&B->d_start, B->layout.start.size() * sizeof(int)) != hipSuccess) {
+    if (dev_malloc((void**)&B->d_descs, nd * sizeof(BatchDesc)) != hipSuccess ||
+        dev_malloc((void**)&B->d_start, B->layout.start.size() * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();
       return fail("descriptor allocation failed");
     }

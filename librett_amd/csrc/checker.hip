@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "devmem.h"
 #include "../../include/librett_c.h"
 #include "checker.h"
 
@@ -154,11 +155,11 @@ int librett_c_check_transpose(const void* odata, int rank, const int* dim, const
   CheckPartial* d_part = nullptr;
   std::vector<CheckPartial> h(grid);
   hipStream_t s = (hipStream_t)stream;
-  bool ok = hipMalloc((void**)&d_part, grid * sizeof(CheckPartial)) == hipSuccess &&
+  bool ok = dev_malloc((void**)&d_part, grid * sizeof(CheckPartial)) == hipSuccess &&
             launch_check(a, odata, d_part, grid, s) == hipSuccess &&
             hipMemcpyAsync(h.data(), d_part, grid * sizeof(CheckPartial), hipMemcpyDeviceToHost, s) == hipSuccess &&
             hipStreamSynchronize(s) == hipSuccess;
-  if (d_part) (void)hipFree(d_part);
+  if (d_part) (void)dev_free(d_part);
   if (!ok) {
     (void)hipGetLastError();
     return LIBRETT_C_INTERNAL_ERROR;

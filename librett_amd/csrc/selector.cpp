@@ -20,6 +20,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "devmem.h"
 #include "../../include/librett_c.h"
 #include "selector.h"
 #include "selector_ops.h"
@@ -293,11 +294,11 @@ int librett_c_mlp_train(const float* x, const float* y, int n, float* params, in
   const int mb = nblocks * mlp::S;
   int* d_idx = nullptr;
   float *d_part = nullptr, *d_m = nullptr, *d_v = nullptr, *d_loss = nullptr;
-  bool ok = hipMalloc((void**)&d_idx, (size_t)n * sizeof(int)) == hipSuccess &&
-            hipMalloc((void**)&d_part, (size_t)nblocks * mlp::P * sizeof(float)) == hipSuccess &&
-            hipMalloc((void**)&d_m, mlp::P * sizeof(float)) == hipSuccess &&
-            hipMalloc((void**)&d_v, mlp::P * sizeof(float)) == hipSuccess &&
-            hipMalloc((void**)&d_loss, (size_t)nblocks * sizeof(float)) == hipSuccess;
+  bool ok = dev_malloc((void**)&d_idx, (size_t)n * sizeof(int)) == hipSuccess &&
+            dev_malloc((void**)&d_part, (size_t)nblocks * mlp::P * sizeof(float)) == hipSuccess &&
+            dev_malloc((void**)&d_m, mlp::P * sizeof(float)) == hipSuccess &&
+            dev_malloc((void**)&d_v, mlp::P * sizeof(float)) == hipSuccess &&
+            dev_malloc((void**)&d_loss, (size_t)nblocks * sizeof(float)) == hipSuccess;
   std::vector<float> zeros(mlp::P, 0.0f);
   ok = ok && hipMemcpyAsync(d_m, zeros.data(), mlp::P * sizeof(float), hipMemcpyHostToDevice, stream) == hipSuccess &&
        hipMemcpyAsync(d_v, zeros.data(), mlp::P * sizeof(float), hipMemcpyHostToDevice, stream) == hipSuccess &&
@@ -332,11 +333,11 @@ int librett_c_mlp_train(const float* x, const float* y, int n, float* params, in
     if (epoch_loss) epoch_loss[ep] = (float)(sse / n);
   }
   ok = ok && hipStreamSynchronize(stream) == hipSuccess;
-  if (d_idx) (void)hipFree(d_idx);
-  if (d_part) (void)hipFree(d_part);
-  if (d_m) (void)hipFree(d_m);
-  if (d_v) (void)hipFree(d_v);
-  if (d_loss) (void)hipFree(d_loss);
+  if (d_idx) (void)dev_free(d_idx);
+  if (d_part) (void)dev_free(d_part);
+  if (d_m) (void)dev_free(d_m);
+  if (d_v) (void)dev_free(d_v);
+  if (d_loss) (void)dev_free(d_loss);
   if (!ok) {
     (void)hipGetLastError();
     return LIBRETT_C_INTERNAL_ERROR;

@@ -35,6 +35,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "devmem.h"
 #include "../../include/librett_c.h"
 #include "batch.h"
 #include "lru_cache.h"
@@ -53,7 +54,7 @@ class Arena {
  public:
   bool init(size_t bytes) {
     cap_ = (bytes + 255) / 256 * 256;
-    if (hipMalloc(&base_, cap_) != hipSuccess) {
+    if (dev_malloc(&base_, cap_) != hipSuccess) {
       (void)hipGetLastError();
       base_ = nullptr;
       return false;
@@ -62,7 +63,7 @@ class Arena {
     return true;
   }
   ~Arena() {
-    if (base_) (void)hipFree(base_);
+    if (base_) (void)dev_free(base_);
   }
   void* alloc(size_t bytes) {
     bytes = std::max<size_t>(256, (bytes + 255) / 256 * 256);
@@ -186,8 +187,8 @@ class Server {
           hipHostMalloc((void**)&s.h_desc, s.cap * sizeof(BatchDesc), hipHostMallocDefault) != hipSuccess ||
           hipHostMalloc((void**)&s.h_start, (s.cap + kBatchFamilies) * sizeof(int), hipHostMallocDefault) !=
               hipSuccess ||
-          hipMalloc((void**)&s.d_desc, s.cap * sizeof(BatchDesc)) != hipSuccess ||
-          hipMalloc((void**)&s.d_start, (s.cap + kBatchFamilies) * sizeof(int)) != hipSuccess) {
+          dev_malloc((void**)&s.d_desc, s.cap * sizeof(BatchDesc)) != hipSuccess ||
+          dev_malloc((void**)&s.d_start, (s.cap + kBatchFamilies) * sizeof(int)) != hipSuccess) {
         (void)hipGetLastError();
         return LIBRETT_C_INTERNAL_ERROR;
       }
@@ -212,8 +213,8 @@ class Server {
       if (s.stream) (void)hipStreamDestroy(s.stream);
       if (s.h_desc) (void)hipHostFree(s.h_desc);
       if (s.h_start) (void)hipHostFree(s.h_start);
-      if (s.d_desc) (void)hipFree(s.d_desc);
-      if (s.d_start) (void)hipFree(s.d_start);
+      if (s.d_desc) (void)dev_free(s.d_desc);
+      if (s.d_start) (void)dev_free(s.d_start);
       s = Slot();
     }
   }

@@ -90,3 +90,16 @@ def test_host_replay_rejects_bad_index():
     x = np.zeros(16, dtype=np.float32)
     with pytest.raises(ValueError):
         L.host_replay([4, 4], [1, 0], 4, 999, x)
+
+
+def test_allocator_hook_arguments():
+    import ctypes
+    lib = L.lib()
+    # alloc without free (and the reverse) is refused; both NULL restores hipMalloc.
+    fa = L.ALLOC_FN(lambda n, c: None)
+    ff = L.FREE_FN(lambda p, c: None)
+    assert lib.librett_c_set_allocator(fa, L.FREE_FN(), None) == L.LIBRETT_INVALID_PARAMETER
+    assert lib.librett_c_set_allocator(L.ALLOC_FN(), ff, None) == L.LIBRETT_INVALID_PARAMETER
+    assert L.set_allocator(lambda n: 0, lambda p: None) == L.LIBRETT_SUCCESS
+    assert L.allocator_live() == 0
+    assert L.set_allocator() == L.LIBRETT_SUCCESS
