@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--pmc-json", default=None, help="with --pmc-summary: also write the summary as JSON here")
     ap.add_argument("--requests", type=int, default=20000, help="serving workload: arrivals per rate")
     ap.add_argument("--arena-gb", type=float, default=64.0, help="serving workload: device arena budget (GiB)")
+    ap.add_argument("--slots", type=int, default=4, help="serving workload: micro-batches in flight")
+    ap.add_argument("--max-batch", type=int, default=64, help="serving workload: requests per micro-batch")
     ap.add_argument("--problems", type=int, default=4096, help="batched workload: number of problems")
     ap.add_argument("--tune", type=int, default=3, help="batched workload: autotuner candidates per problem")
     args = ap.parse_args()
@@ -168,11 +170,13 @@ def main():
     # HBM bytes per launch of the dominant kernel from the committed counter
     # summary, when it was taken with the same plan.
     dom_traffic = None
+    matched = []
     for c in cases:
         if c["info"]["method_name"] + ":" + c["info"]["desc"].split(" grid")[0] == dom_name:
             t = measured_traffic(c["dim"], c["perm"], c["elem"])
-            if t and t.get("plan") == c["info"]["desc"]:
-                dom_traffic = t["traffic_bytes"]
+            matched.append(t["traffic_bytes"] if t and t.get("plan") == c["info"]["desc"] else None)
+    if matched and all(m is not None for m in matched):
+        dom_traffic = int(round(sum(matched) / len(matched)))  # every matched case runs once per step
 
     # Ceilings measured the same way: each transpose of a step replaced by a
     # copy of the same bytes between the same buffers, for the same steps.
@@ -502,7 +506,7 @@ def bench_serving(args, stream):
     pool_bytes = 1 << 30
     pool = hip.DeviceBuffer(pool_bytes)
     hip.memset(pool.ptr, 5, pool_bytes)
-    srv = L.Server(int(args.arena_gb * (1 << 30)), slots=4, max_batch=64)
+    srv = L.Server(int(args.arena_gb * (1 << 30)), slots=args.slots, max_batch=args.max_batch)
     kw = dict(catalog=1024, min_bytes=4096, max_bytes=64 << 20, rank_lo=2, rank_hi=6, elems=(4, 8, 16))
     # Capacity: a rate far above what the device sustains (closed-loop saturation).
     srv.run_poisson(pool.ptr, pool_bytes, qps=1e7, requests=2000, seed=1, **kw)  # warm-up
@@ -567,7 +571,8 @@ def bench_serving(args, stream):
         "dtype": "f32/f64/c128 (opaque moves)",
         "data": "synthetic",
         "config": {"workload": "serving: Poisson arrivals, 4 KiB-64 MiB log-uniform, rank 2-6, f32/f64/c128, "
-                               "%g GiB arena, LRU plan cache, catalog of 1024 shapes" % args.arena_gb},
+                               "%g GiB arena, LRU plan cache, catalog of 1024 shapes" % args.arena_gb,
+                   "slots": args.slots, "max_batch": args.max_batch},
         "saturation": {"gbs": round(sat["gbs"], 1), "req_per_s": round(cap_qps), "p50_us": round(sat["p50_us"], 1),
                        "p99_us": round(sat["p99_us"], 1), "mean_batch": round(sat["mean_batch"], 2)},
         "load_sweep": rates,

@@ -59,7 +59,8 @@ Selector& state() {
 
 constexpr uint32_t kMagic = 0x4D54524C;  // "LRTM"
 
-int set_weights(const float* params, int nparams, const float* mean, const float* stdv, int nfeat) {
+// Installs weights (params NULL: none).
+int install_weights(const float* params, int nparams, const float* mean, const float* stdv, int nfeat) {
   Selector& s = state();
   std::lock_guard<std::mutex> lock(s.mu);
   if (!params) {
@@ -77,7 +78,8 @@ int set_weights(const float* params, int nparams, const float* mean, const float
   return LIBRETT_C_SUCCESS;
 }
 
-int load_weights(const char* path) {
+// Reads and installs a weights file.
+int install_file(const char* path) {
   if (!path) return LIBRETT_C_INVALID_PARAMETER;
   FILE* fp = std::fopen(path, "rb");
   if (!fp) return LIBRETT_C_INVALID_PARAMETER;
@@ -90,7 +92,7 @@ int load_weights(const char* path) {
   ok = ok && std::fread(params.data(), sizeof(float), mlp::P, fp) == (size_t)mlp::P;
   std::fclose(fp);
   if (!ok) return LIBRETT_C_INVALID_PARAMETER;
-  return set_weights(params.data(), mlp::P, mean.data(), stdv.data(), mlp::F);
+  return install_weights(params.data(), mlp::P, mean.data(), stdv.data(), mlp::F);
 }
 
 // Directory of the shared library that holds this code.
@@ -111,7 +113,7 @@ void load_from_env() {
     const char* p = std::getenv("LIBRETT_SELECTOR_WEIGHTS");
     if (p) {
       if (!*p || std::strcmp(p, "none") == 0) return;
-      if (load_weights(p) != LIBRETT_C_SUCCESS)
+      if (install_file(p) != LIBRETT_C_SUCCESS)
         std::fprintf(stderr, "librett: cannot load selector weights from %s; using the analytic model\n", p);
       return;
     }
@@ -119,9 +121,21 @@ void load_from_env() {
     FILE* fp = std::fopen(def.c_str(), "rb");
     if (!fp) return;
     std::fclose(fp);
-    if (load_weights(def.c_str()) != LIBRETT_C_SUCCESS)
+    if (install_file(def.c_str()) != LIBRETT_C_SUCCESS)
       std::fprintf(stderr, "librett: cannot load selector weights from %s; using the analytic model\n", def.c_str());
   });
+}
+
+// The run-time setters: the first-use load happens before them, so what
+// they install replaces it instead of being replaced by it.
+int set_weights(const float* params, int nparams, const float* mean, const float* stdv, int nfeat) {
+  load_from_env();
+  return install_weights(params, nparams, mean, stdv, nfeat);
+}
+
+int load_weights(const char* path) {
+  load_from_env();
+  return install_file(path);
 }
 
 }  // namespace

@@ -167,3 +167,15 @@ def test_records_roundtrip_without_features(tmp_path):
     # Candidates that did not run stay marked, and the targets skip them.
     y = S.targets(back[0])
     assert np.isnan(y[1]) and np.all(np.isfinite(y[[0, 2]]))
+
+
+def test_unloading_first_keeps_analytic_model(tmp_path):
+    # In a fresh process, clearing the weights before anything else must not
+    # be undone by the first-use load of the default weights file.
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r); import librett_amd as L; "
+            "assert L.selector_set(None, None, None) == 0; print(L.selector_generation())") % S.__file__.rsplit("/", 2)[0]
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "0"
