@@ -68,7 +68,7 @@ def main():
     ap.add_argument("--pmc-json", default=None, help="with --pmc-summary: also write the summary as JSON here")
     ap.add_argument("--requests", type=int, default=20000, help="serving workload: arrivals per rate")
     ap.add_argument("--arena-gb", type=float, default=64.0, help="serving workload: device arena budget (GiB)")
-    ap.add_argument("--slots", type=int, default=4, help="serving workload: micro-batches in flight")
+    ap.add_argument("--slots", type=int, default=8, help="serving workload: micro-batches in flight")
     ap.add_argument("--max-batch", type=int, default=64, help="serving workload: requests per micro-batch")
     ap.add_argument("--problems", type=int, default=4096, help="batched workload: number of problems")
     ap.add_argument("--tune", type=int, default=3, help="batched workload: autotuner candidates per problem")

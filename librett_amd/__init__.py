@@ -410,7 +410,7 @@ class Batch:
 class Server:
     """Continuous-batching serving engine (extension of the API)."""
 
-    def __init__(self, arena_bytes, slots=4, max_batch=64):
+    def __init__(self, arena_bytes, slots=8, max_batch=64):
         h = ctypes.c_uint(0)
         r = lib().librett_c_server_create(ctypes.byref(h), int(arena_bytes), int(slots), int(max_batch))
         if r != LIBRETT_SUCCESS:
