@@ -220,6 +220,9 @@ struct PackedEntry {
 static const TiledEntry kTiledTable[] = {
     LRT_TILED_V(4, 64, 64, 4) LRT_TILED_V(4, 32, 32, 4) LRT_TILED_V(4, 64, 32, 4) LRT_TILED_V(4, 32, 64, 4)
     LRT_TILED_V(8, 32, 32, 2) LRT_TILED_V(8, 64, 64, 2) LRT_TILED_V(8, 64, 32, 2) LRT_TILED_V(8, 32, 64, 2)
+    // Tiles with 512-byte row segments (66 KiB of LDS for the largest).
+    LRT_TILED_V(4, 128, 128, 4) LRT_TILED_V(4, 128, 64, 4) LRT_TILED_V(4, 64, 128, 4)
+    LRT_TILED_V(8, 128, 64, 2) LRT_TILED_V(8, 64, 128, 2)
     LRT_TILED(16, 16, 16, 1, 1) LRT_TILED(16, 32, 32, 1, 1) LRT_TILED(16, 32, 16, 1, 1) LRT_TILED(16, 16, 32, 1, 1)
 };
 

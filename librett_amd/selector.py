@@ -351,15 +351,16 @@ def main(argv=None):
         p, mean, std, losses = train(recs, stream, a.epochs, a.lr, log=log)
         t2 = time.time()
         save_weights(a.weights, p, mean, std)
-        A, B = _buffers(512 << 20)
-        ho = collect(problems(a.heldout, a.seed + 1000), stream, A.ptr, B.ptr, log)
         s_tr, a_tr = evaluate(recs, scores(p, mean, std, recs, stream))
-        s_ho, a_ho = evaluate(ho, scores(p, mean, std, ho, stream))
-        rep = {"train_problems": len(recs), "heldout_problems": len(ho),
-               "samples": int(sum(len(r["times"]) for r in recs)), "epochs": a.epochs, "lr": a.lr,
-               "final_mse": float(losses[-1]), "train_s": t2 - t1, "seed": a.seed,
-               "train": {"selector": summary(s_tr), "analytic": summary(a_tr)},
-               "heldout": {"selector": summary(s_ho), "analytic": summary(a_ho)}}
+        rep = {"train_problems": len(recs), "samples": int(sum(len(r["times"]) for r in recs)),
+               "epochs": a.epochs, "lr": a.lr, "final_mse": float(losses[-1]), "train_s": t2 - t1, "seed": a.seed,
+               "train": {"selector": summary(s_tr), "analytic": summary(a_tr)}}
+        if a.heldout > 0:  # --heldout 0 leaves the held-out evaluation to a run of its own
+            A, B = _buffers(512 << 20)
+            ho = collect(problems(a.heldout, a.seed + 1000), stream, A.ptr, B.ptr, log)
+            s_ho, a_ho = evaluate(ho, scores(p, mean, std, ho, stream))
+            rep["heldout_problems"] = len(ho)
+            rep["heldout"] = {"selector": summary(s_ho), "analytic": summary(a_ho)}
         print(json.dumps(rep))
         if a.report:
             with open(a.report, "w") as fh:
