@@ -289,26 +289,29 @@ unsigned long long librett_c_selector_generation(void);
 
 /* MLP kernels on device pointers. forward: out[n] = MLP(x[n][50]).
  * grad: workgroup b (of nblocks) writes the gradient of samples
- * idx[base + 128 b .. min(end, base + 128 (b+1))) of sum (out - y)^2 * inv_m
- * to partial[b][nparams] and its squared-error sum to loss[b].
+ * idx[base + 128 b .. min(end, base + 128 (b+1))) of sum w (out - y)^2 * inv_m
+ * (w: per-sample weights, NULL for 1) to partial[b][nparams] and its weighted
+ * squared-error sum to loss[b].
  * adam: params -= lr * mhat / (sqrt(vhat) + eps) with the gradient summed
  * over the nparts partials; c1 = 1/(1-beta1^t), c2 = 1/(1-beta2^t).
  * train: Adam over `epochs` shuffled passes (std::mt19937(seed)) with
- * minibatches of `minibatch` samples (a multiple of 128); epoch_loss (host,
- * may be NULL) gets the mean squared error of each epoch. */
+ * minibatches of `minibatch` samples (a multiple of 128), each normalised by
+ * its weight sum; w (device) and w_host (the same weights on the host) are
+ * both NULL for unweighted training; epoch_loss (host, may be NULL) gets the
+ * weighted mean squared error of each epoch. */
 int librett_c_mlp_forward(const float* params, const float* x, int n, float* out, void* stream);
-int librett_c_mlp_grad(const float* params, const float* x, const float* y, const int* idx, int base, int end,
-                       float inv_m, int nblocks, float* partial, float* loss, void* stream);
+int librett_c_mlp_grad(const float* params, const float* x, const float* y, const float* w, const int* idx, int base,
+                       int end, float inv_m, int nblocks, float* partial, float* loss, void* stream);
 int librett_c_mlp_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr, float beta1,
                        float beta2, float c1, float c2, float eps, void* stream);
-int librett_c_mlp_train(const float* x, const float* y, int n, float* params, int epochs, float lr, int minibatch,
-                        unsigned int seed, float* epoch_loss, void* stream);
+int librett_c_mlp_train(const float* x, const float* y, const float* w, const float* w_host, int n, float* params,
+                        int epochs, float lr, int minibatch, unsigned int seed, float* epoch_loss, void* stream);
 
 /* Host replays of the same kernels' per-thread code (one workgroup for
  * grad: count <= 128 samples idx[0..count)). */
 void librett_c_mlp_host_forward(const float* params, const float* x, int n, float* out);
-int librett_c_mlp_host_grad(const float* params, const float* x, const float* y, const int* idx, int count,
-                            float inv_m, float* grad, float* loss);
+int librett_c_mlp_host_grad(const float* params, const float* x, const float* y, const float* w, const int* idx,
+                            int count, float inv_m, float* grad, float* loss);
 void librett_c_mlp_host_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr,
                              float beta1, float beta2, float c1, float c2, float eps);
 

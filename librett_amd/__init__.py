@@ -212,12 +212,12 @@ def lib():
         L.librett_c_selector_load.argtypes = [ctypes.c_char_p]
         L.librett_c_selector_generation.argtypes = []
         L.librett_c_mlp_forward.argtypes = [vp, vp, ctypes.c_int, vp, vp]
-        L.librett_c_mlp_grad.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, f_, ctypes.c_int, vp, vp, vp]
+        L.librett_c_mlp_grad.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, f_, ctypes.c_int, vp, vp, vp]
         L.librett_c_mlp_adam.argtypes = [vp, ctypes.c_int, vp, vp, vp, f_, f_, f_, f_, f_, f_, vp]
-        L.librett_c_mlp_train.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, f_, ctypes.c_int, ctypes.c_uint, fp,
-                                          vp]
+        L.librett_c_mlp_train.argtypes = [vp, vp, vp, fp, ctypes.c_int, vp, ctypes.c_int, f_, ctypes.c_int,
+                                          ctypes.c_uint, fp, vp]
         L.librett_c_mlp_host_forward.argtypes = [fp, fp, ctypes.c_int, fp]
-        L.librett_c_mlp_host_grad.argtypes = [fp, fp, fp, ip, ctypes.c_int, f_, fp, fp]
+        L.librett_c_mlp_host_grad.argtypes = [fp, fp, fp, fp, ip, ctypes.c_int, f_, fp, fp]
         L.librett_c_mlp_host_adam.argtypes = [fp, ctypes.c_int, fp, fp, fp, f_, f_, f_, f_, f_, f_]
         llp = ctypes.POINTER(ctypes.c_longlong)
         L.librett_c_check_fill.argtypes = [vp, ctypes.c_size_t, vp]
@@ -589,17 +589,19 @@ def mlp_host_forward(params, x):
     return out
 
 
-def mlp_host_grad(params, x, y, idx, inv_m):
-    """Host replay of one training workgroup: (gradient, squared-error sum)."""
+def mlp_host_grad(params, x, y, idx, inv_m, w=None):
+    """Host replay of one training workgroup: (gradient, weighted squared-error sum)."""
     import numpy as np
     p = np.ascontiguousarray(params, dtype=np.float32)
     xx = np.ascontiguousarray(x, dtype=np.float32)
     yy = np.ascontiguousarray(y, dtype=np.float32)
+    ww = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
     ii = np.ascontiguousarray(idx, dtype=np.int32)
     g = np.zeros(p.size, dtype=np.float32)
     loss = np.zeros(1, dtype=np.float32)
-    r = lib().librett_c_mlp_host_grad(_fptr(p), _fptr(xx), _fptr(yy), ii.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                      ii.size, float(inv_m), _fptr(g), _fptr(loss))
+    r = lib().librett_c_mlp_host_grad(_fptr(p), _fptr(xx), _fptr(yy), None if ww is None else _fptr(ww),
+                                      ii.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ii.size, float(inv_m), _fptr(g),
+                                      _fptr(loss))
     if r != LIBRETT_SUCCESS:
         raise ValueError("mlp_host_grad failed: %d" % r)
     return g, float(loss[0])
@@ -624,8 +626,8 @@ def mlp_forward(d_params, d_x, n, d_out, stream):
     _ok(lib().librett_c_mlp_forward(d_params, d_x, n, d_out, ctypes.c_void_p(stream)), "mlp_forward")
 
 
-def mlp_grad(d_params, d_x, d_y, d_idx, base, end, inv_m, nblocks, d_partial, d_loss, stream):
-    _ok(lib().librett_c_mlp_grad(d_params, d_x, d_y, d_idx, base, end, inv_m, nblocks, d_partial, d_loss,
+def mlp_grad(d_params, d_x, d_y, d_idx, base, end, inv_m, nblocks, d_partial, d_loss, stream, d_w=None):
+    _ok(lib().librett_c_mlp_grad(d_params, d_x, d_y, d_w, d_idx, base, end, inv_m, nblocks, d_partial, d_loss,
                                  ctypes.c_void_p(stream)), "mlp_grad")
 
 
@@ -634,12 +636,15 @@ def mlp_adam(d_partial, nparts, d_params, d_m, d_v, lr, beta1, beta2, c1, c2, ep
                                  ctypes.c_void_p(stream)), "mlp_adam")
 
 
-def mlp_train(d_x, d_y, n, d_params, epochs, lr, minibatch, seed, stream):
-    """Trains on the GPU; returns the per-epoch mean squared error."""
+def mlp_train(d_x, d_y, n, d_params, epochs, lr, minibatch, seed, stream, d_w=None, w_host=None):
+    """Trains on the GPU; returns the per-epoch (weighted) mean squared error.
+    d_w / w_host: per-sample weights on the device and the same on the host
+    (numpy float32), both or neither."""
     import numpy as np
     loss = np.zeros(max(1, epochs), dtype=np.float32)
-    _ok(lib().librett_c_mlp_train(d_x, d_y, n, d_params, epochs, lr, minibatch, seed, _fptr(loss),
-                                  ctypes.c_void_p(stream)), "mlp_train")
+    wh = None if w_host is None else np.ascontiguousarray(w_host, dtype=np.float32)
+    _ok(lib().librett_c_mlp_train(d_x, d_y, d_w, None if wh is None else _fptr(wh), n, d_params, epochs, lr, minibatch,
+                                  seed, _fptr(loss), ctypes.c_void_p(stream)), "mlp_train")
     return loss[:epochs]
 
 

@@ -276,12 +276,12 @@ int librett_c_mlp_forward(const float* params, const float* x, int n, float* out
   return LIBRETT_C_SUCCESS;
 }
 
-int librett_c_mlp_grad(const float* params, const float* x, const float* y, const int* idx, int base, int end,
-                       float inv_m, int nblocks, float* partial, float* loss, void* stream) {
+int librett_c_mlp_grad(const float* params, const float* x, const float* y, const float* w, const int* idx, int base,
+                       int end, float inv_m, int nblocks, float* partial, float* loss, void* stream) {
   if (!params || !x || !y || !idx || !partial || !loss || nblocks < 1 || base < 0 || end < base)
     return LIBRETT_C_INVALID_PARAMETER;
   if (mlp::prepare_grad() != hipSuccess ||
-      mlp::launch_grad(params, x, y, idx, base, end, inv_m, nblocks, partial, loss, (hipStream_t)stream) !=
+      mlp::launch_grad(params, x, y, w, idx, base, end, inv_m, nblocks, partial, loss, (hipStream_t)stream) !=
           hipSuccess) {
     (void)hipGetLastError();
     return LIBRETT_C_INTERNAL_ERROR;
@@ -300,9 +300,10 @@ int librett_c_mlp_adam(const float* partial, int nparts, float* params, float* m
   return LIBRETT_C_SUCCESS;
 }
 
-int librett_c_mlp_train(const float* x, const float* y, int n, float* params, int epochs, float lr, int minibatch,
-                        unsigned int seed, float* epoch_loss, void* stream_) {
+int librett_c_mlp_train(const float* x, const float* y, const float* w, const float* w_host, int n, float* params,
+                        int epochs, float lr, int minibatch, unsigned int seed, float* epoch_loss, void* stream_) {
   if (!x || !y || !params || n < 1 || epochs < 0 || minibatch < mlp::S) return LIBRETT_C_INVALID_PARAMETER;
+  if ((w == nullptr) != (w_host == nullptr)) return LIBRETT_C_INVALID_PARAMETER;
   hipStream_t stream = (hipStream_t)stream_;
   const int nblocks = minibatch / mlp::S;
   const int mb = nblocks * mlp::S;
@@ -327,14 +328,21 @@ int librett_c_mlp_train(const float* x, const float* y, int n, float* params, in
     std::shuffle(order.begin(), order.end(), rng);
     ok = hipMemcpyAsync(d_idx, order.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, stream) == hipSuccess &&
          hipStreamSynchronize(stream) == hipSuccess;
-    double sse = 0;
+    double sse = 0, wsum = 0;
     for (int base = 0; ok && base < n; base += mb) {
       const int end = std::min(n, base + mb);
       step++;
       const float c1 = (float)(1.0 / (1.0 - std::pow(beta1, (double)step)));
       const float c2 = (float)(1.0 / (1.0 - std::pow(beta2, (double)step)));
-      ok = mlp::launch_grad(params, x, y, d_idx, base, end, 1.0f / (float)(end - base), nblocks, d_part, d_loss,
-                            stream) == hipSuccess &&
+      // The minibatch's weight sum normalises its loss.
+      double m = end - base;
+      if (w_host) {
+        m = 0;
+        for (int k = base; k < end; k++) m += w_host[order[k]];
+      }
+      wsum += m;
+      ok = mlp::launch_grad(params, x, y, w, d_idx, base, end, (float)(1.0 / std::max(m, 1e-30)), nblocks, d_part,
+                            d_loss, stream) == hipSuccess &&
            mlp::launch_adam(d_part, nblocks, params, d_m, d_v, lr, (float)beta1, (float)beta2, c1, c2, 1e-8f,
                             stream) == hipSuccess;
       if (ok && epoch_loss) {
@@ -344,7 +352,7 @@ int librett_c_mlp_train(const float* x, const float* y, int n, float* params, in
         for (float l : lossv) sse += l;
       }
     }
-    if (epoch_loss) epoch_loss[ep] = (float)(sse / n);
+    if (epoch_loss) epoch_loss[ep] = (float)(sse / std::max(wsum, 1e-30));
   }
   ok = ok && hipStreamSynchronize(stream) == hipSuccess;
   if (d_idx) (void)dev_free(d_idx);
@@ -363,10 +371,10 @@ void librett_c_mlp_host_forward(const float* params, const float* x, int n, floa
   mlp::host_forward(params, x, n, out);
 }
 
-int librett_c_mlp_host_grad(const float* params, const float* x, const float* y, const int* idx, int count,
-                            float inv_m, float* grad, float* loss) {
+int librett_c_mlp_host_grad(const float* params, const float* x, const float* y, const float* w, const int* idx,
+                            int count, float inv_m, float* grad, float* loss) {
   if (!params || !x || !y || !idx || !grad || !loss || count < 0 || count > mlp::S) return LIBRETT_C_INVALID_PARAMETER;
-  mlp::host_grad(params, x, y, idx, count, inv_m, grad, loss);
+  mlp::host_grad(params, x, y, w, idx, count, inv_m, grad, loss);
   return LIBRETT_C_SUCCESS;
 }
 

@@ -15,14 +15,14 @@ namespace mlp {
 
 hipError_t launch_forward(const float* params, const float* xs, int n, float* out, hipStream_t stream);
 hipError_t prepare_grad();
-hipError_t launch_grad(const float* params, const float* xs, const float* ys, const int* idx, int base, int end,
-                       float inv_m, int nblocks, float* partial, float* loss, hipStream_t stream);
+hipError_t launch_grad(const float* params, const float* xs, const float* ys, const float* ws, const int* idx,
+                       int base, int end, float inv_m, int nblocks, float* partial, float* loss, hipStream_t stream);
 hipError_t launch_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr, float beta1,
                        float beta2, float c1, float c2, float eps, hipStream_t stream);
 
 void host_forward(const float* params, const float* xs, int n, float* out);
-void host_grad(const float* params, const float* xs, const float* ys, const int* idx, int count, float inv_m,
-               float* grad, float* loss);
+void host_grad(const float* params, const float* xs, const float* ys, const float* ws, const int* idx, int count,
+               float inv_m, float* grad, float* loss);
 void host_adam(const float* partial, int nparts, float* params, float* m, float* v, float lr, float beta1,
                float beta2, float c1, float c2, float eps);
 

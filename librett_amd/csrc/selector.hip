@@ -27,9 +27,9 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ p
 // Training: workgroup b computes the gradient of samples
 // idx[base + b*S .. base + b*S + count) into partial[b].
 __global__ __launch_bounds__(T) void k_mlp_grad(const float* __restrict__ params, const float* __restrict__ xs,
-                                                const float* __restrict__ ys, const int* __restrict__ idx, int base,
-                                                int end, float inv_m, float* __restrict__ partial,
-                                                float* __restrict__ loss) {
+                                                const float* __restrict__ ys, const float* __restrict__ ws,
+                                                const int* __restrict__ idx, int base, int end, float inv_m,
+                                                float* __restrict__ partial, float* __restrict__ loss) {
   extern __shared__ float lds[];
   const int first = base + blockIdx.x * S;
   const int count = max(0, min(S, end - first));
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(T) void k_mlp_grad(const float* __restrict__ params
   const int t = threadIdx.x;
   phase_load(t, lds, params, xs, id, count);
   __syncthreads();
-  phase_forward(t, lds, ys, id, count, inv_m);
+  phase_forward(t, lds, ys, ws, id, count, inv_m);
   __syncthreads();
   phase_w3(t, lds, grad, loss + blockIdx.x);
   __syncthreads();
@@ -71,9 +71,9 @@ hipError_t prepare_grad() {
                              kLdsFloats * (int)sizeof(float));
 }
 
-hipError_t launch_grad(const float* params, const float* xs, const float* ys, const int* idx, int base, int end,
-                       float inv_m, int nblocks, float* partial, float* loss, hipStream_t stream) {
-  void* args[9] = {&params, &xs, &ys, &idx, &base, &end, &inv_m, &partial, &loss};
+hipError_t launch_grad(const float* params, const float* xs, const float* ys, const float* ws, const int* idx,
+                       int base, int end, float inv_m, int nblocks, float* partial, float* loss, hipStream_t stream) {
+  void* args[10] = {&params, &xs, &ys, &ws, &idx, &base, &end, &inv_m, &partial, &loss};
   return hipLaunchKernel(reinterpret_cast<const void*>(&k_mlp_grad), dim3(nblocks), dim3(T), args,
                          kLdsFloats * sizeof(float), stream);
 }
@@ -94,12 +94,12 @@ void host_forward(const float* params, const float* xs, int n, float* out) {
   for (int s = 0; s < n; s++) out[s] = forward(params, xs + (long long)s * F);
 }
 
-void host_grad(const float* params, const float* xs, const float* ys, const int* idx, int count, float inv_m,
-               float* grad, float* loss) {
+void host_grad(const float* params, const float* xs, const float* ys, const float* ws, const int* idx, int count,
+               float inv_m, float* grad, float* loss) {
   std::vector<float> lds(kLdsFloats);
   float* L = lds.data();
   for (int t = 0; t < T; t++) phase_load(t, L, params, xs, idx, count);
-  for (int t = 0; t < T; t++) phase_forward(t, L, ys, idx, count, inv_m);
+  for (int t = 0; t < T; t++) phase_forward(t, L, ys, ws, idx, count, inv_m);
   for (int t = 0; t < T; t++) phase_w3(t, L, grad, loss);
   for (int t = 0; t < T; t++) phase_dz2(t, L);
   for (int t = 0; t < T; t++) phase_w2(t, L, grad);

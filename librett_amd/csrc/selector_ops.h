@@ -64,8 +64,9 @@ __host__ __device__ inline float forward(const float* __restrict__ p, const floa
 
 // ---------------------------------------------------------------------------
 // Training phases. One workgroup handles `count` (<= S) samples whose rows
-// are idx[0..count); the loss is sum (out - y)^2 * inv_m over the minibatch,
-// so the workgroup's gradient is its share of the minibatch gradient.
+// are idx[0..count); the loss is sum w (out - y)^2 * inv_m over the
+// minibatch (w = 1 when no weights are given; inv_m = 1 / sum w), so the
+// workgroup's gradient is its share of the minibatch gradient.
 // ---------------------------------------------------------------------------
 
 // Phase 0: parameters and the workgroup's inputs into LDS.
@@ -81,7 +82,8 @@ __host__ __device__ inline void phase_load(int tid, float* lds, const float* __r
 // Phase 1: forward pass per sample; keeps pre-activations, d loss / d out
 // and the squared error.
 __host__ __device__ inline void phase_forward(int tid, float* lds, const float* __restrict__ ys,
-                                              const int* __restrict__ idx, int count, float inv_m) {
+                                              const float* __restrict__ ws, const int* __restrict__ idx, int count,
+                                              float inv_m) {
   const float* p = lds + kLdsP;
   for (int s = tid; s < S; s += T) {
     float* z1 = lds + kLdsZ1 + s * H;
@@ -106,12 +108,13 @@ __host__ __device__ inline void phase_forward(int tid, float* lds, const float* 
       out += p[kW3 + j] * relu(z);
     }
     const float err = out - ys[idx[s]];
-    lds[kLdsD + s] = 2.0f * err * inv_m;
-    lds[kLdsE + s] = err * err;
+    const float w = ws ? ws[idx[s]] : 1.0f;
+    lds[kLdsD + s] = 2.0f * w * err * inv_m;
+    lds[kLdsE + s] = w * err * err;
   }
 }
 
-// Phase 2: gradients of w3 and b3, and the workgroup's squared-error sum.
+// Phase 2: gradients of w3 and b3, and the workgroup's weighted squared-error sum.
 __host__ __device__ inline void phase_w3(int tid, const float* lds, float* __restrict__ grad,
                                          float* __restrict__ loss) {
   for (int j = tid; j < H + 2; j += T) {

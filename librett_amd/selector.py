@@ -193,14 +193,24 @@ def targets(rec):
     return y
 
 
+# Sample weights: a candidate within a factor 2^tau of its problem's fastest
+# counts fully, slower ones fade to the floor; selection only needs the
+# order near the top.
+WEIGHT_TAU = 0.5
+WEIGHT_FLOOR = 0.1
+
+
 def dataset(recs):
-    xs, ys = [], []
+    """(features, targets, weights) of every candidate that ran."""
+    xs, ys, ws = [], [], []
     for r in recs:
         y = targets(r)
         ok = np.isfinite(y)
         xs.append(r["feats"][ok])
         ys.append(y[ok])
-    return np.concatenate(xs).astype(np.float32), np.concatenate(ys).astype(np.float32)
+        ws.append(np.exp(-(y[ok] - y[ok].min()) / WEIGHT_TAU) + WEIGHT_FLOOR)
+    return (np.concatenate(xs).astype(np.float32), np.concatenate(ys).astype(np.float32),
+            np.concatenate(ws).astype(np.float32))
 
 
 def normaliser(x):
@@ -213,24 +223,27 @@ def normaliser(x):
 def train(recs, stream, epochs=60, lr=2e-3, minibatch=2048, seed=1, log=None):
     """Trains on the GPU; returns (params, mean, std, per-epoch loss)."""
     from . import hip, mlp_train
-    x, y = dataset(recs)
+    x, y, w = dataset(recs)
     mean, std = normaliser(x)
     xn = ((x - mean) / std).astype(np.float32)
     p = init_params(seed)
-    dx, dy, dp = hip.DeviceBuffer(xn.nbytes), hip.DeviceBuffer(y.nbytes), hip.DeviceBuffer(p.nbytes)
+    dx, dy, dw, dp = (hip.DeviceBuffer(xn.nbytes), hip.DeviceBuffer(y.nbytes), hip.DeviceBuffer(w.nbytes),
+                      hip.DeviceBuffer(p.nbytes))
     hip.htod(dx.ptr, xn)
     hip.htod(dy.ptr, y)
+    hip.htod(dw.ptr, w)
     hip.htod(dp.ptr, p)
     losses = []
     chunk = 10
     for e0 in range(0, epochs, chunk):
         n_ep = min(chunk, epochs - e0)
-        losses += list(mlp_train(dx.ptr, dy.ptr, len(y), dp.ptr, n_ep, lr, minibatch, seed + e0, stream))
+        losses += list(mlp_train(dx.ptr, dy.ptr, len(y), dp.ptr, n_ep, lr, minibatch, seed + e0, stream,
+                                 d_w=dw.ptr, w_host=w))
         if log:
-            log("epoch %d: mse %.4f" % (e0 + n_ep, losses[-1]))
+            log("epoch %d: weighted mse %.4f" % (e0 + n_ep, losses[-1]))
     hip.stream_sync(stream)
     hip.dtoh(p, dp.ptr)
-    for b in (dx, dy, dp):
+    for b in (dx, dy, dw, dp):
         b.free()
     return p, mean, std, np.array(losses)
 

@@ -12,7 +12,7 @@ width 64, one linear output,
     z2 = W2 a1 + b2, a2 = relu(z2)
     out = w3 . a2 + b3
 
-with the loss ``inv_m * sum_s (out_s - y_s)^2`` and the Adam update
+with the loss ``inv_m * sum_s w_s (out_s - y_s)^2`` (w = 1 when unweighted) and the Adam update
 (Kingma & Ba, 2015, Algorithm 1) with bias corrections ``c1 = 1/(1 -
 beta1^t)``, ``c2 = 1/(1 - beta2^t)`` applied as
 ``p -= lr * (m c1) / (sqrt(v c2) + eps)``.
@@ -55,13 +55,14 @@ def forward(p, x):
     return a2 @ w3 + b3
 
 
-def loss(p, x, y, inv_m):
+def loss(p, x, y, inv_m, w=None):
     out = forward(p, x)
-    return inv_m * np.sum((out - np.asarray(y, dtype=np.float64)) ** 2)
+    w = 1.0 if w is None else np.asarray(w, dtype=np.float64)
+    return inv_m * np.sum(w * (out - np.asarray(y, dtype=np.float64)) ** 2)
 
 
-def grad(p, x, y, inv_m):
-    """(gradient of loss(p, x, y, inv_m) w.r.t. p, sum of squared errors)."""
+def grad(p, x, y, inv_m, w=None):
+    """(gradient of loss(p, x, y, inv_m, w) w.r.t. p, weighted sum of squared errors)."""
     W1, b1, W2, b2, w3, b3 = unpack(p)
     x = np.asarray(x, dtype=np.float64)
     y = np.asarray(y, dtype=np.float64)
@@ -71,7 +72,8 @@ def grad(p, x, y, inv_m):
     a2 = np.maximum(z2, 0.0)
     out = a2 @ w3 + b3
     err = out - y
-    d = 2.0 * err * inv_m                       # d loss / d out
+    w = np.ones_like(err) if w is None else np.asarray(w, dtype=np.float64)
+    d = 2.0 * w * err * inv_m                   # d loss / d out
     gw3 = a2.T @ d
     gb3 = d.sum()
     dz2 = np.outer(d, w3) * (z2 > 0)
@@ -80,7 +82,7 @@ def grad(p, x, y, inv_m):
     dz1 = (dz2 @ W2) * (z1 > 0)
     gW1 = dz1.T @ x
     gb1 = dz1.sum(axis=0)
-    return pack(gW1, gb1, gW2, gb2, gw3, gb3), float(np.sum(err * err))
+    return pack(gW1, gb1, gW2, gb2, gw3, gb3), float(np.sum(w * err * err))
 
 
 def adam(p, m, v, g, lr, beta1, beta2, t, eps):

@@ -63,18 +63,23 @@ def test_gpu_grad_several_workgroups(gpu_stream):
     base, end, nb = 100, 550, 4
     dp, dx, dy, di = dev_array(p), dev_array(x), dev_array(y), dev_array(idx)
     dpart, dloss = hip.DeviceBuffer(nb * M.P * 4), hip.DeviceBuffer(nb * 4)
-    L.mlp_grad(dp.ptr, dx.ptr, dy.ptr, di.ptr, base, end, 1.0 / (end - base), nb, dpart.ptr, dloss.ptr, gpu_stream)
-    hip.stream_sync(gpu_stream)
-    part = fetch(dpart, np.float32, nb * M.P).reshape(nb, M.P)
-    loss = fetch(dloss, np.float32, nb)
     sel = idx[base:end]
-    gr, sse = M.grad(p, x[sel], y[sel], 1.0 / (end - base))
-    g = part.astype(np.float64).sum(0)
-    assert np.all(np.abs(g - gr) <= 1e-4 * np.abs(gr).max() + 1e-7)
-    assert abs(loss.sum() - sse) <= 1e-4 * (1 + sse)
-    # The last workgroup's share (samples 384..449 of the minibatch) equals the host replay.
-    hg, _ = L.mlp_host_grad(p, x, y, sel[384:], 1.0 / (end - base))
-    assert np.all(np.abs(part[3] - hg) <= 1e-5 * np.abs(hg).max() + 1e-8)
+    w = (rng.random(n) + 0.1).astype(np.float32)
+    dw = dev_array(w)
+    for weighted in (False, True):
+        inv_m = 1.0 / (float(w[sel].astype(np.float64).sum()) if weighted else (end - base))
+        L.mlp_grad(dp.ptr, dx.ptr, dy.ptr, di.ptr, base, end, inv_m, nb, dpart.ptr, dloss.ptr, gpu_stream,
+                   d_w=dw.ptr if weighted else None)
+        hip.stream_sync(gpu_stream)
+        part = fetch(dpart, np.float32, nb * M.P).reshape(nb, M.P)
+        loss = fetch(dloss, np.float32, nb)
+        gr, sse = M.grad(p, x[sel], y[sel], inv_m, w[sel] if weighted else None)
+        g = part.astype(np.float64).sum(0)
+        assert np.all(np.abs(g - gr) <= 1e-4 * np.abs(gr).max() + 1e-7)
+        assert abs(loss.sum() - sse) <= 1e-4 * (1 + sse)
+        # The last workgroup's share (samples 384..449 of the minibatch) equals the host replay.
+        hg, _ = L.mlp_host_grad(p, x, y, sel[384:], inv_m, w=w if weighted else None)
+        assert np.all(np.abs(part[3] - hg) <= 1e-5 * np.abs(hg).max() + 1e-8)
 
 
 def test_gpu_adam(gpu_stream):
@@ -116,6 +121,12 @@ def test_gpu_training_fits_a_teacher(gpu_stream):
     assert np.all(np.isfinite(losses))
     # 30 epochs from a He initialisation explain most of the variance.
     assert losses[-1] < 0.3 * y.var() and losses[-1] < 0.2 * losses[0]
+    # Weighted training (weights on the device and the host) converges too.
+    w = (np.random.default_rng(6).random(len(y)) + 0.1).astype(np.float32)
+    dw, dq = dev_array(w), dev_array(S.init_params(8))
+    lw = L.mlp_train(dx.ptr, dy.ptr, len(y), dq.ptr, 30, 2e-3, 1024, 5, gpu_stream, d_w=dw.ptr, w_host=w)
+    say("  weighted mse by epoch", lw[0], lw[-1])
+    assert np.all(np.isfinite(lw)) and lw[-1] < 0.2 * lw[0]
     q = fetch(dp, np.float32, M.P)
     out = hip.DeviceBuffer(len(y) * 4)
     L.mlp_forward(dp.ptr, dx.ptr, len(y), out.ptr, gpu_stream)

@@ -52,6 +52,26 @@ def test_backward_finite_differences():
         assert abs(fd - g[i]) <= 1e-5 * (1 + abs(g[i])), (i, fd, g[i])
 
 
+def test_weighted_backward_finite_differences():
+    rng = np.random.default_rng(7)
+    p = rand_params(3).astype(np.float64)
+    x = rng.normal(size=(16, M.F))
+    y = rng.normal(size=16)
+    w = rng.random(16) + 0.1
+    inv_m = 1 / w.sum()
+    g, sse = M.grad(p, x, y, inv_m, w)
+    assert abs(sse - np.sum(w * (M.forward(p, x) - y) ** 2)) < 1e-9 * (1 + sse)
+    h = 1e-6
+    for i in rng.choice(M.P, 40, replace=False):
+        e = np.zeros(M.P); e[i] = h
+        fd = (M.loss(p + e, x, y, inv_m, w) - M.loss(p - e, x, y, inv_m, w)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-5 * (1 + abs(g[i])), (i, fd, g[i])
+    # Unit weights are the unweighted loss.
+    g1, _ = M.grad(p, x, y, 1 / 16, np.ones(16))
+    g0, _ = M.grad(p, x, y, 1 / 16)
+    assert np.allclose(g1, g0)
+
+
 def test_adam_first_step_closed_form():
     rng = np.random.default_rng(3)
     g = rng.normal(size=M.P)
@@ -71,16 +91,18 @@ def test_host_forward_matches_restatement():
     assert np.array_equal(np.argmin(got.reshape(12, 25), 1), np.argmin(ref.reshape(12, 25), 1))
 
 
-@pytest.mark.parametrize("count", [128, 77, 1])
-def test_host_grad_matches_restatement(count):
+@pytest.mark.parametrize("count,weighted", [(128, False), (77, False), (1, False), (128, True), (77, True)])
+def test_host_grad_matches_restatement(count, weighted):
     rng = np.random.default_rng(count)
     p = rand_params(5)
     n = 400
     x = rng.normal(size=(n, M.F)).astype(np.float32)
     y = rng.normal(size=n).astype(np.float32)
+    w = (rng.random(n) + 0.1).astype(np.float32) if weighted else None
     idx = rng.choice(n, count, replace=False).astype(np.int32)
-    g, sse = L.mlp_host_grad(p, x, y, idx, 1.0 / count)
-    gr, sser = M.grad(p, x[idx], y[idx], 1.0 / count)
+    inv_m = 1.0 / (float(w[idx].astype(np.float64).sum()) if weighted else count)
+    g, sse = L.mlp_host_grad(p, x, y, idx, inv_m, w=w)
+    gr, sser = M.grad(p, x[idx], y[idx], inv_m, None if w is None else w[idx])
     scale = np.abs(gr).max()
     assert np.all(np.abs(g - gr) <= 1e-4 * scale + 1e-7), np.abs(g - gr).max() / scale
     assert abs(sse - sser) <= 1e-4 * (1 + sser)
