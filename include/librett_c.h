@@ -225,7 +225,9 @@ int librett_c_server_free(unsigned int handle, void* ptr);
 /* Queues one request; returns at once with a ticket. */
 int librett_c_server_submit(unsigned int handle, int rank, const int* dim, const int* permutation,
                             size_t sizeofType, const void* idata, void* odata, unsigned long long* ticket);
-/* Blocks until the request is complete; latency = completion - submission. */
+/* Blocks until the request is complete; latency = completion - submission.
+ * A finished request can still be waited on until 2^20 newer requests have
+ * finished or the statistics are reset. */
 int librett_c_server_wait(unsigned int handle, unsigned long long ticket, double* latency_s);
 int librett_c_server_drain(unsigned int handle);
 int librett_c_server_stats_get(unsigned int handle, librett_c_server_stats* stats);

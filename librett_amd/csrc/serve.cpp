@@ -313,6 +313,7 @@ class Server {
     first_arrival_ = 1e300;
     last_done_ = 0;
     finished_.clear();
+    finished_order_.clear();
     arena_.reset_peak();
   }
 
@@ -417,7 +418,14 @@ class Server {
       first_arrival_ = std::min(first_arrival_, r->t_arrival);
       last_done_ = std::max(last_done_, t);
       if (r->release_out) arena_.release(r->out);
+      // Latencies of finished requests stay available to wait() for the
+      // most recent kFinishedKeep of them.
       finished_[r->id] = t - r->t_arrival;
+      finished_order_.push_back(r->id);
+      if (finished_order_.size() > kFinishedKeep) {
+        finished_.erase(finished_order_.front());
+        finished_order_.pop_front();
+      }
       live_.erase(r->id);
     }
     completions_ += s.reqs.size();
@@ -479,7 +487,9 @@ class Server {
   std::condition_variable cv_, done_cv_;
   std::deque<std::shared_ptr<Request>> pending_;
   std::unordered_map<unsigned long long, std::shared_ptr<Request>> live_;
+  static constexpr size_t kFinishedKeep = 1 << 20;
   std::unordered_map<unsigned long long, double> finished_;
+  std::deque<unsigned long long> finished_order_;
   std::thread dispatcher_;
   bool stop_ = true;
   unsigned long long next_id_ = 1, completions_ = 0;
