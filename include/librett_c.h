@@ -123,6 +123,50 @@ int librett_c_reduce_ranks(int rank, const int* dim, const int* permutation, int
 int librett_c_device_copy(void* dst, const void* src, size_t bytes, void* stream, int words_per_thread,
                           int nontemporal, int grid);
 
+/* ---------------------------------------------------------------------
+ * Access counters (extension). Reference: librettGpuModelKernel and its
+ * MemStat counters, src/GpuModelKernel.cpp:43-64, 343-946, 1013-1224.
+ * Exact counts over every work item of a candidate's launch, computed on
+ * the device by a kernel that forms the same lane addresses as the
+ * transpose kernel (the planner's model estimates them from sampled work
+ * items). A work item is a tile (Tiled, TiledCopy) or one staged round,
+ * i.e. one chunk at one outer index (Packed, PackedSplit). Trivial plans
+ * (a device copy) have no kernel to count: LIBRETT_C_INVALID_PARAMETER.
+ * ------------------------------------------------------------------- */
+typedef struct librett_c_counters {
+  unsigned long long ld_req, st_req;     /* wave load / store instructions with an active lane */
+  unsigned long long ld_tran, st_tran;   /* distinct 128-byte segments per instruction, summed */
+  unsigned long long ld_bytes, st_bytes; /* bytes the active lanes load / store (vol * elem each) */
+  unsigned long long read_lines;         /* distinct 128-byte lines each work item reads, summed */
+  unsigned long long write_lines;        /* distinct 128-byte lines each work item writes, summed */
+  unsigned long long partial_lines;      /* written lines a work item covers only in part, summed */
+  unsigned long long items;              /* work items */
+} librett_c_counters;
+
+/* Counts of a stored plan's kernel, on the current device (synchronises
+ * the plan's stream). */
+int librett_c_plan_counters(unsigned int handle, librett_c_counters* out);
+
+/* Counts of candidate `index` of librett_c_candidates, on the current
+ * device; synchronises `stream`. */
+int librett_c_candidate_counters(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                                 void* stream, int index, librett_c_counters* out);
+
+/* The same counts computed on the host by the same per-item code, for
+ * candidate `index` of librett_c_host_candidates. */
+int librett_c_host_candidate_counters(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                                      const librett_c_device* dev, int index, librett_c_counters* out);
+
+/* Host-only access trace of candidate `index` of librett_c_host_candidates,
+ * six values per access: work item, 0 load / 1 store, instruction, thread,
+ * byte offset from the tensor start, bytes. source 0: the counters'
+ * enumeration (instruction = pass); source 1: the transpose kernel's host
+ * replay (instruction = call order within the thread). At most `max`
+ * records are written; *count receives the total. */
+int librett_c_host_access_trace(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                                const librett_c_device* dev, int index, int source, long long* records,
+                                long long max, long long* count);
+
 /* Extension: overrides the analytic model's constants, in this order:
  * sustained HBM fraction, written-line weight, partial-line weight, round
  * latency (s), launch overhead (s), seconds per load transaction per CU,

@@ -82,8 +82,19 @@ EXPORTED = [
     "librett_c_mlp_host_grad", "librett_c_mlp_host_adam",
     "librett_c_check_fill", "librett_c_check_transpose", "librett_c_host_check_fill", "librett_c_host_check_transpose",
     "librett_c_set_allocator", "librett_c_allocator_live",
+    "librett_c_plan_counters", "librett_c_candidate_counters", "librett_c_host_candidate_counters",
+    "librett_c_host_access_trace",
 ]
 LIBRETT_BUSY = 6
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_ulonglong) for n in (
+        "ld_req", "st_req", "ld_tran", "st_tran", "ld_bytes", "st_bytes", "read_lines", "write_lines",
+        "partial_lines", "items")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class ServerStats(ctypes.Structure):
@@ -226,6 +237,13 @@ def lib():
         L.librett_c_host_check_transpose.argtypes = [vp, ctypes.c_int, ip, ip, ctypes.c_size_t, ullp, llp]
         L.librett_c_set_allocator.argtypes = [ALLOC_FN, FREE_FN, vp]
         L.librett_c_allocator_live.argtypes = []
+        cp = ctypes.POINTER(Counters)
+        L.librett_c_plan_counters.argtypes = [ctypes.c_uint, cp]
+        L.librett_c_candidate_counters.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, vp, ctypes.c_int, cp]
+        L.librett_c_host_candidate_counters.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t,
+                                                        ctypes.POINTER(DeviceDesc), ctypes.c_int, cp]
+        L.librett_c_host_access_trace.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, ctypes.POINTER(DeviceDesc),
+                                                  ctypes.c_int, ctypes.c_int, llp, ctypes.c_longlong, llp]
         L.librett_c_version.restype = ctypes.c_char_p
         for name in EXPORTED:
             f = getattr(L, name)
@@ -679,6 +697,50 @@ def host_check_transpose(arr, dim, permutation, sizeofType):
     if r != LIBRETT_SUCCESS:
         raise ValueError("host_check_transpose failed: %d" % r)
     return n.value, f.value
+
+
+# ---------------------------------------------------------------------------
+# Access counters (extension; counterpart of the reference's
+# librettGpuModelKernel)
+# ---------------------------------------------------------------------------
+
+def plan_counters(handle):
+    """Exact access counts of a stored plan's kernel, on the device."""
+    c = Counters()
+    _ok(lib().librett_c_plan_counters(ctypes.c_uint(handle), ctypes.byref(c)), "plan_counters")
+    return c.as_dict()
+
+
+def candidate_counters(dim, permutation, sizeofType, index, stream=0):
+    """Exact access counts of candidate `index` of candidates(), on the device."""
+    c = Counters()
+    _ok(lib().librett_c_candidate_counters(len(dim), _iarr(dim), _iarr(permutation), int(sizeofType),
+                                           ctypes.c_void_p(stream), int(index), ctypes.byref(c)), "candidate_counters")
+    return c.as_dict()
+
+
+def host_candidate_counters(dim, permutation, sizeofType, index, device=None):
+    """The same counts from the host run of the counting code (host_candidates() indices)."""
+    dev = device or mi355x_desc()
+    c = Counters()
+    _ok(lib().librett_c_host_candidate_counters(len(dim), _iarr(dim), _iarr(permutation), int(sizeofType),
+                                                ctypes.byref(dev), int(index), ctypes.byref(c)),
+        "host_candidate_counters")
+    return c.as_dict()
+
+
+def host_access_trace(dim, permutation, sizeofType, index, source, device=None):
+    """(n, 6) int64 array of accesses (item, store, instruction, thread, byte offset, bytes).
+    source 0: the counters' enumeration; 1: the kernel's host replay."""
+    import numpy as np
+    dev = device or mi355x_desc()
+    n = ctypes.c_longlong(0)
+    args = (len(dim), _iarr(dim), _iarr(permutation), int(sizeofType), ctypes.byref(dev), int(index), int(source))
+    _ok(lib().librett_c_host_access_trace(*args, None, 0, ctypes.byref(n)), "host_access_trace")
+    out = np.zeros((max(1, n.value), 6), dtype=np.int64)
+    _ok(lib().librett_c_host_access_trace(*args, out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), n.value,
+                                          ctypes.byref(n)), "host_access_trace")
+    return out[:n.value]
 
 
 def version():

@@ -27,6 +27,7 @@
 
 #include "../../include/librett.h"
 #include "../../include/librett_c.h"
+#include "counters.h"
 #include "kernels.h"
 #include "lru_cache.h"
 #include "planner.h"
@@ -600,5 +601,105 @@ int librett_c_device_copy(void* dst, const void* src, size_t bytes, void* stream
 void librett_c_set_model_constants(const double* values, int count) { set_model_constants(values, count); }
 
 const char* librett_c_version(void) { return "librett-mi355x 0.1 (gfx950)"; }
+
+static void copy_counts(const AccessCounts& a, librett_c_counters* out) {
+  out->ld_req = a.v[kLdReq];
+  out->st_req = a.v[kStReq];
+  out->ld_tran = a.v[kLdTran];
+  out->st_tran = a.v[kStTran];
+  out->ld_bytes = a.v[kLdBytes];
+  out->st_bytes = a.v[kStBytes];
+  out->read_lines = a.v[kReadLines];
+  out->write_lines = a.v[kWriteLines];
+  out->partial_lines = a.v[kPartialLines];
+  out->items = a.v[kItems];
+}
+
+static int counts_of(const Candidate& c, hipStream_t stream, librett_c_counters* out) {
+  if (c.method == kTrivial) return LIBRETT_INVALID_PARAMETER;
+  AccessCounts a;
+  const hipError_t e = device_counts(c, stream, a);
+  if (e == hipErrorInvalidValue) return LIBRETT_INVALID_PARAMETER;
+  if (e != hipSuccess) return LIBRETT_INTERNAL_ERROR;
+  copy_counts(a, out);
+  return LIBRETT_SUCCESS;
+}
+
+int librett_c_plan_counters(unsigned int handle, librett_c_counters* out) {
+  if (!out) return LIBRETT_INVALID_PARAMETER;
+  std::shared_ptr<PlanObj> p = find_plan(handle);
+  if (!p) return LIBRETT_INVALID_PLAN;
+  try {
+    return counts_of(p->cand, p->stream, out);
+  } catch (...) {
+    return LIBRETT_INTERNAL_ERROR;
+  }
+}
+
+int librett_c_candidate_counters(int rank, const int* dim, const int* permutation, size_t sizeofType, void* stream,
+                                 int index, librett_c_counters* out) {
+  librettResult r = check_input(rank, dim, permutation, sizeofType);
+  if (r != LIBRETT_SUCCESS) return r;
+  if (!out) return LIBRETT_INVALID_PARAMETER;
+  try {
+    DeviceInfo dev;
+    if (!device_info(dev)) return LIBRETT_INTERNAL_ERROR;
+    const Problem full = canonicalize(rank, dim, permutation, (int)sizeofType);
+    const std::vector<Candidate> cands = make_candidates(full, dev);
+    if (index < 0 || index >= (int)cands.size()) return LIBRETT_INVALID_PARAMETER;
+    return counts_of(cands[index], (hipStream_t)stream, out);
+  } catch (...) {
+    return LIBRETT_INTERNAL_ERROR;
+  }
+}
+
+int librett_c_host_candidate_counters(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                                      const librett_c_device* d, int index, librett_c_counters* out) {
+  librettResult r = check_input(rank, dim, permutation, sizeofType);
+  if (r != LIBRETT_SUCCESS) return r;
+  if (!out) return LIBRETT_INVALID_PARAMETER;
+  try {
+    const DeviceInfo dev = host_device(d);
+    const Problem full = canonicalize(rank, dim, permutation, (int)sizeofType);
+    const std::vector<Candidate> cands = make_candidates(full, dev);
+    if (index < 0 || index >= (int)cands.size() || cands[index].method == kTrivial) return LIBRETT_INVALID_PARAMETER;
+    AccessCounts a;
+    if (!host_counts(cands[index], a)) return LIBRETT_INVALID_PARAMETER;
+    copy_counts(a, out);
+    return LIBRETT_SUCCESS;
+  } catch (...) {
+    return LIBRETT_INTERNAL_ERROR;
+  }
+}
+
+int librett_c_host_access_trace(int rank, const int* dim, const int* permutation, size_t sizeofType,
+                                const librett_c_device* d, int index, int source, long long* records,
+                                long long max, long long* count) {
+  librettResult r = check_input(rank, dim, permutation, sizeofType);
+  if (r != LIBRETT_SUCCESS) return r;
+  if (!count || (source != 0 && source != 1)) return LIBRETT_INVALID_PARAMETER;
+  try {
+    const DeviceInfo dev = host_device(d);
+    const Problem full = canonicalize(rank, dim, permutation, (int)sizeofType);
+    const std::vector<Candidate> cands = make_candidates(full, dev);
+    if (index < 0 || index >= (int)cands.size() || cands[index].method == kTrivial) return LIBRETT_INVALID_PARAMETER;
+    std::vector<AccessRecord> recs;
+    const bool ok = source == 0 ? trace_enumeration(cands[index], recs) : trace_kernel(cands[index], recs);
+    if (!ok) return LIBRETT_INTERNAL_ERROR;
+    *count = (long long)recs.size();
+    for (long long i = 0; i < (long long)recs.size() && i < max && records; i++) {
+      long long* q = records + 6 * i;
+      q[0] = recs[i].item;
+      q[1] = recs[i].store;
+      q[2] = recs[i].inst;
+      q[3] = recs[i].thread;
+      q[4] = recs[i].offset;
+      q[5] = recs[i].bytes;
+    }
+    return LIBRETT_SUCCESS;
+  } catch (...) {
+    return LIBRETT_INTERNAL_ERROR;
+  }
+}
 
 }  // extern "C"

@@ -45,19 +45,39 @@ __host__ __device__ inline void vset(V& v, int k, const E& e) {
 #ifndef LRT_NT_LOADS
 #define LRT_NT_LOADS 0
 #endif
+// Host replay only: an observer that is told which work item and thread the
+// replay is running and sees every global load and store it makes. The
+// tests of the access counters compare these accesses with the counters'
+// own enumeration. Null (the normal case) costs one test per access.
+struct ReplayObserver {
+  virtual void at(long long item, int thread) = 0;
+  virtual void access(const void* p, int bytes, int store) = 0;
+};
+extern thread_local ReplayObserver* g_replay_observer;
+
 template <typename V>
 __host__ __device__ inline void st(V* p, const V& v) {
-#if defined(__HIP_DEVICE_COMPILE__) && LRT_NT_STORES
+#if defined(__HIP_DEVICE_COMPILE__)
+#if LRT_NT_STORES
   __builtin_nontemporal_store(v, p);
 #else
+  *p = v;
+#endif
+#else
+  if (g_replay_observer) g_replay_observer->access(p, (int)sizeof(V), 1);
   *p = v;
 #endif
 }
 template <typename V>
 __host__ __device__ inline V ld(const V* p) {
-#if defined(__HIP_DEVICE_COMPILE__) && LRT_NT_LOADS
+#if defined(__HIP_DEVICE_COMPILE__)
+#if LRT_NT_LOADS
   return __builtin_nontemporal_load(p);
 #else
+  return *p;
+#endif
+#else
+  if (g_replay_observer) g_replay_observer->access(p, (int)sizeof(V), 0);
   return *p;
 #endif
 }

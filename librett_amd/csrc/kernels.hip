@@ -131,6 +131,12 @@ hipError_t launch_device_copy(void* dst, const void* src, size_t bytes, hipStrea
 // thread's load() before any thread's store().
 // ---------------------------------------------------------------------------
 
+thread_local ReplayObserver* g_replay_observer = nullptr;
+
+static inline void replay_at(long long item, int t) {
+  if (g_replay_observer) g_replay_observer->at(item, t);
+}
+
 template <int B, int TX, int TY, int VX, int VY>
 static void replay_tiled(const Candidate& c, const void* in, void* out) {
   using Ops = TiledOps<B, TX, TY, VX, VY>;
@@ -139,8 +145,14 @@ static void replay_tiled(const Candidate& c, const void* in, void* out) {
   const TiledArgs& a = c.tiled;
   for (uint32_t w = 0; w < (uint32_t)a.items; w++) {
     const TileCtx ctx = Ops::item(a, w);
-    for (int t = 0; t < Ops::NT; t++) Ops::load(a, ctx, t, static_cast<const E*>(in), lds.data());
-    for (int t = 0; t < Ops::NT; t++) Ops::store(a, ctx, t, static_cast<E*>(out), lds.data());
+    for (int t = 0; t < Ops::NT; t++) {
+      replay_at(w, t);
+      Ops::load(a, ctx, t, static_cast<const E*>(in), lds.data());
+    }
+    for (int t = 0; t < Ops::NT; t++) {
+      replay_at(w, t);
+      Ops::store(a, ctx, t, static_cast<E*>(out), lds.data());
+    }
   }
 }
 
@@ -151,8 +163,10 @@ static void replay_copy(const Candidate& c, const void* in, void* out) {
   const CopyArgs& a = c.copy;
   for (uint32_t w = 0; w < (uint32_t)a.items; w++) {
     const TileCtx ctx = Ops::item(a, w);
-    for (int t = 0; t < Ops::NT; t++)
+    for (int t = 0; t < Ops::NT; t++) {
+      replay_at(w, t);
       Ops::run(a, ctx, t, static_cast<const V*>(in), static_cast<V*>(out));
+    }
   }
 }
 
@@ -184,10 +198,15 @@ static void replay_packed(const Candidate& c, const void* in, void* out) {
     for (int o = o0; o < o1; o += step) {
       int off_in, off_out;
       outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
-      for (int t = 0; t < nt; t++)
+      const long long item = (long long)split * a.vol_outer + o;
+      for (int t = 0; t < nt; t++) {
+        replay_at(item, t);
         Ops::load(pre[t], off_in, t, nt, static_cast<const E*>(in), lds.data());
-      for (int t = 0; t < nt; t++)
+      }
+      for (int t = 0; t < nt; t++) {
+        replay_at(item, t);
         Ops::store(pre[t], off_out, t, nt, static_cast<E*>(out), lds.data());
+      }
     }
   }
 }

@@ -765,8 +765,8 @@ void model_copy(Candidate& c) {
   c.model.steps_per_block = (double)a.items / c.grid_x;
 }
 
-// Contiguous runs (start offset, length in elements) of one staged shape
-// walked in LDS order (input side) or output order (output side).
+}  // namespace
+
 void staged_runs(const StagedShape& s, bool output_side, std::vector<std::pair<long long, int>>& runs) {
   runs.clear();
   const int n = s.n;
@@ -796,6 +796,8 @@ void staged_runs(const StagedShape& s, bool output_side, std::vector<std::pair<l
   }
   if (run_len > 0) runs.push_back({run_start, run_len});
 }
+
+namespace {
 
 // Packed / PackedSplit: lines are counted over a window of consecutive outer
 // indices of one block, because a block walks its outer range in order and

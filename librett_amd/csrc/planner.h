@@ -9,6 +9,7 @@
 #pragma once
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -133,6 +134,10 @@ void set_tile_group(TiledArgs& a, int nty, int group);
 // Variant of a Tiled candidate without vector accesses, used when the
 // caller's pointers are not aligned to the vector width.
 Candidate scalar_fallback(const Candidate& c);
+
+// Contiguous runs (start offset, length in elements) of one staged shape
+// walked in LDS order (input side) or output order (output side).
+void staged_runs(const StagedShape& s, bool output_side, std::vector<std::pair<long long, int>>& runs);
 
 // Active blocks per CU for a candidate's kernel (runtime occupancy query if
 // a GPU is present, otherwise an LDS / wave-slot estimate).
