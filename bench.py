@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--measure", action="store_true", help="choose plans with librettPlanMeasure")
     ap.add_argument("--analytic", action="store_true",
                     help="librettPlan with the analytic model only (no learned selector)")
+    ap.add_argument("--exact-counts", action="store_true",
+                    help="LIBRETT_EXACT_COUNTS=1: librettPlan refines the model's sampled line counts with the "
+                         "device counters")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--workload", default="large",
@@ -80,11 +83,15 @@ def main():
               "multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.exact_counts:
+        os.environ["LIBRETT_EXACT_COUNTS"] = "1"
     import librett_amd as L
     from librett_amd import data, hip
 
     hip.check(hip.runtime().hipSetDevice(0), "hipSetDevice")
     stream = hip.stream_create()
+    if args.analytic:
+        L.selector_set(None, None, None)
     if args.workload == "batched":
         return bench_batched(args, stream)
     if args.workload == "serving":
@@ -93,10 +100,10 @@ def main():
         return bench_selector(args, stream)
     if args.workload == "large-random":
         return bench_large_random(args, stream)
-    if args.analytic:
-        L.selector_set(None, None, None)
     selection = ("librettPlanMeasure" if args.measure else
                  "learned selector (librettPlan)" if L.selector_generation() else "gfx950 analytic model (librettPlan)")
+    if args.exact_counts and not args.measure:
+        selection += " with exact line counts"
 
     # One random source buffer, uploaded as the input of every case.
     max_bytes = max(int(np.prod(d)) * e for d, _, e in CASES)
@@ -363,13 +370,17 @@ def bench_selector(args, stream):
     t1 = time.perf_counter()
     sc = S.scores(p, mean, std, recs, stream)
     infer = time.perf_counter() - t1
-    sel, ana = S.evaluate(recs, sc)
-    ss, sa = S.summary(sel), S.summary(ana)
+    margin = L.selector_margin()
+    sel, ana = S.evaluate(recs, sc, margin)
+    pure, _ = S.evaluate(recs, sc)
+    ss, sa, sp = S.summary(sel), S.summary(ana), S.summary(pure)
     ncand = sum(len(r["times"]) for r in recs)
-    log("selector: mean slowdown %.4f (p50 %.3f p90 %.3f p99 %.3f, %.1f%% within 5%%); analytic %.4f (p50 %.3f "
-        "p90 %.3f p99 %.3f, %.1f%% within 5%%); %d problems, %d candidates timed in %.0f s; inference %.1f ms"
-        % (ss["mean"], ss["p50"], ss["p90"], ss["p99"], 100 * ss["within_5pct"], sa["mean"], sa["p50"], sa["p90"],
-           sa["p99"], 100 * sa["within_5pct"], len(recs), ncand, coll, infer * 1e3))
+    log("librettPlan (selector, margin %.2f): mean slowdown %.4f (p50 %.3f p90 %.3f p99 %.3f, %.1f%% within 5%%); "
+        "selector alone %.4f (p90 %.3f, %.1f%% within 5%%); analytic %.4f (p50 %.3f p90 %.3f p99 %.3f, %.1f%% within "
+        "5%%); %d problems, %d candidates timed in %.0f s; inference %.1f ms"
+        % (margin, ss["mean"], ss["p50"], ss["p90"], ss["p99"], 100 * ss["within_5pct"], sp["mean"], sp["p90"],
+           100 * sp["within_5pct"], sa["mean"], sa["p50"], sa["p90"], sa["p99"], 100 * sa["within_5pct"], len(recs),
+           ncand, coll, infer * 1e3))
     line = {
         "metric": "plan selection: mean slowdown of the chosen plan vs the exhaustive best (held-out problems)",
         "value": round(ss["mean"], 4), "unit": "x", "n_gpus": 1, "steps": len(recs), "warmup": 0,
@@ -377,7 +388,8 @@ def bench_selector(args, stream):
         "vs_baseline": None, "dtype": "f32 (MLP); f32/f64/c128 opaque moves timed", "data": "synthetic",
         "config": {"workload": "learned selector: %d held-out problems, rank 2-7, 1 MB-512 MB, f32/f64/c128"
                                % len(recs), "seed": 1001, "candidates_timed": ncand},
-        "selector": ss, "analytic": sa, "inference_ms_all_candidates": round(infer * 1e3, 2),
+        "selector": ss, "selector_margin": margin, "selector_alone": sp, "analytic": sa,
+        "inference_ms_all_candidates": round(infer * 1e3, 2),
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

@@ -323,15 +323,22 @@ int librett_c_selector_choose(int rank, const int* dim, const int* permutation, 
 
 /* Loads weights: nparams floats in the layout W1[64][50], b1[64], W2[64][64],
  * b2[64], w3[64], b3, with per-feature mean and std used to normalise the
- * inputs. params NULL unloads. */
+ * inputs. nparams may be a multiple k (k <= 16) of one network's count: an
+ * ensemble of k networks whose scores are averaged. params NULL unloads. */
 int librett_c_selector_set(const float* params, int nparams, const float* mean, const float* std, int nfeat);
 
 /* Loads a weights file: uint32 magic 0x4D54524C, version 1, 50, 64, then
- * float32 mean[50], std[50], params[nparams] (little-endian). */
+ * float32 mean[50], std[50], params[nparams] (little-endian); version 2
+ * has uint32 k (networks) after the 64 and k * nparams parameters. */
 int librett_c_selector_load(const char* path);
 
 /* Generation of the loaded weights (0: none). */
 unsigned long long librett_c_selector_generation(void);
+
+/* librettPlan keeps the analytic model's choice unless the selector scores
+ * its own pick lower by more than this margin (log2 of predicted time; 0.1
+ * by default, LIBRETT_SELECTOR_MARGIN overrides). */
+double librett_c_selector_margin(void);
 
 /* MLP kernels on device pointers. forward: out[n] = MLP(x[n][50]).
  * grad: workgroup b (of nblocks) writes the gradient of samples

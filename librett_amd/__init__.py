@@ -83,7 +83,7 @@ EXPORTED = [
     "librett_c_check_fill", "librett_c_check_transpose", "librett_c_host_check_fill", "librett_c_host_check_transpose",
     "librett_c_set_allocator", "librett_c_allocator_live",
     "librett_c_plan_counters", "librett_c_candidate_counters", "librett_c_host_candidate_counters",
-    "librett_c_host_access_trace",
+    "librett_c_host_access_trace", "librett_c_selector_margin",
 ]
 LIBRETT_BUSY = 6
 
@@ -244,6 +244,7 @@ def lib():
                                                         ctypes.POINTER(DeviceDesc), ctypes.c_int, cp]
         L.librett_c_host_access_trace.argtypes = [ctypes.c_int, ip, ip, ctypes.c_size_t, ctypes.POINTER(DeviceDesc),
                                                   ctypes.c_int, ctypes.c_int, llp, ctypes.c_longlong, llp]
+        L.librett_c_selector_margin.argtypes = []
         L.librett_c_version.restype = ctypes.c_char_p
         for name in EXPORTED:
             f = getattr(L, name)
@@ -254,6 +255,7 @@ def lib():
                      "librett_c_host_check_fill"):
             getattr(L, name).restype = None
         L.librett_c_selector_generation.restype = ctypes.c_ulonglong
+        L.librett_c_selector_margin.restype = ctypes.c_double
         L.librett_c_allocator_live.restype = ctypes.c_ulonglong
         L.librett_c_initialize.restype = None
         L.librett_c_finalize.restype = None
@@ -570,6 +572,12 @@ def selector_features(dim, permutation, sizeofType, device=None, max_count=1024)
     if r != LIBRETT_SUCCESS:
         raise ValueError("selector_features failed: %d" % r)
     return f[:min(n.value, max_count)].copy(), mc.value
+
+
+def selector_margin():
+    """Margin (log2 of predicted time) by which the selector must beat the
+    analytic model's choice for librettPlan to take its pick."""
+    return lib().librett_c_selector_margin()
 
 
 def selector_choose(dim, permutation, sizeofType):

@@ -327,6 +327,28 @@ void candidate_features(const Problem& full, const std::vector<Candidate>& cands
 
 // Shared body of librettPlan / librettPlanMeasure / the forced-candidate plan.
 // mode 0: model, 1: measure, 2: candidate `index`.
+// LIBRETT_EXACT_COUNTS=1: before a model (or selector) choice, the line
+// counts of Tiled and TiledCopy candidates whose model sampled only some of
+// their tiles are replaced by the device counters' exact ones (the model
+// counts lines the same way for these methods) and their times recomputed.
+// Costs one counting launch per such candidate at plan time.
+bool exact_counts_enabled() {
+  static const bool on = env_int("LIBRETT_EXACT_COUNTS", 0) != 0;
+  return on;
+}
+
+void refine_counts(std::vector<Candidate>& cands, const DeviceInfo& dev, hipStream_t stream) {
+  for (Candidate& c : cands) {
+    if ((c.method != kTiled && c.method != kTiledCopy) || c.model.items <= 24) continue;
+    AccessCounts a;
+    if (device_counts(c, stream, a) != hipSuccess) continue;
+    c.model.read_lines = (double)a.v[kReadLines];
+    c.model.write_lines = (double)a.v[kWriteLines];
+    c.model.partial_lines = (double)a.v[kPartialLines];
+    model_times(c, dev);
+  }
+}
+
 librettResult plan_impl(unsigned int* handle, int rank, const int* dim, const int* perm, size_t sizeofType,
                         hipStream_t stream, int mode, const void* in, void* out, int index) {
   if (!handle) return LIBRETT_INVALID_PARAMETER;
@@ -349,6 +371,7 @@ librettResult plan_impl(unsigned int* handle, int rank, const int* dim, const in
       if (cands.empty()) return LIBRETT_INTERNAL_ERROR;
       int best = -1;
       if (mode == 0) {
+        if (exact_counts_enabled()) refine_counts(cands, dev, stream);
         // The learned selector when weights are loaded, else (or when it
         // declines the problem) the analytic model.
         best = selector_choose(cands, full, dev.device_id);

@@ -914,6 +914,14 @@ void model_candidate(Candidate& c, const Problem& full, const DeviceInfo& dev) {
       model_packed(c, dev);
       break;
   }
+  model_times(c, dev);
+  (void)full;
+}
+
+void model_times(Candidate& c, const DeviceInfo& dev) {
+  ModelStats& m = c.model;
+  const ModelConst& K = g_model;
+  if (c.method == kTrivial) return;
   const double blocks = (double)c.grid_x * c.grid_y;
   m.concurrency = std::min(blocks, (double)dev.cu_count * std::max(1, c.active_blocks_per_cu));
   const double line_bytes =
@@ -929,7 +937,6 @@ void model_candidate(Candidate& c, const Problem& full, const DeviceInfo& dev) {
   m.t_latency = rounds * (K.round_latency + round_bytes / K.block_bw);
   m.t_launch = K.launch;
   m.predicted_s = std::max(m.t_hbm, m.t_latency) + m.t_launch;
-  (void)full;
 }
 
 std::vector<Candidate> make_candidates(const Problem& full, const DeviceInfo& dev) {

@@ -119,6 +119,10 @@ std::vector<Candidate> make_candidates(const Problem& full, const DeviceInfo& de
 // Applies the gfx950 cost model to a candidate (fills cand.model).
 void model_candidate(Candidate& c, const Problem& full, const DeviceInfo& dev);
 
+// Recomputes the modelled times from the counts already in cand.model (for
+// counts replaced by exact ones).
+void model_times(Candidate& c, const DeviceInfo& dev);
+
 // Overrides the model constants (bw_frac, write_weight, partial_weight,
 // inflight_half, round_latency, launch); entries <= 0 keep their value.
 void set_model_constants(const double* v, int n);

@@ -59,7 +59,9 @@ Selector& state() {
 
 constexpr uint32_t kMagic = 0x4D54524C;  // "LRTM"
 
-// Installs weights (params NULL: none).
+// Installs weights (params NULL: none): one network, or an ensemble of
+// nparams / P networks that share the input normalisation and whose scores
+// are averaged.
 int install_weights(const float* params, int nparams, const float* mean, const float* stdv, int nfeat) {
   Selector& s = state();
   std::lock_guard<std::mutex> lock(s.mu);
@@ -67,7 +69,9 @@ int install_weights(const float* params, int nparams, const float* mean, const f
     s.gen = 0;
     return LIBRETT_C_SUCCESS;
   }
-  if (nparams != mlp::P || nfeat != mlp::F || !mean || !stdv) return LIBRETT_C_INVALID_PARAMETER;
+  if (nparams <= 0 || nparams % mlp::P != 0 || nparams / mlp::P > kSelMaxModels || nfeat != mlp::F || !mean ||
+      !stdv)
+    return LIBRETT_C_INVALID_PARAMETER;
   for (int i = 0; i < nparams; i++)
     if (!std::isfinite(params[i])) return LIBRETT_C_INVALID_PARAMETER;
   s.params.assign(params, params + nparams);
@@ -84,15 +88,20 @@ int install_file(const char* path) {
   FILE* fp = std::fopen(path, "rb");
   if (!fp) return LIBRETT_C_INVALID_PARAMETER;
   uint32_t hdr[4] = {0, 0, 0, 0};
-  std::vector<float> mean(mlp::F), stdv(mlp::F), params(mlp::P);
-  bool ok = std::fread(hdr, sizeof(uint32_t), 4, fp) == 4 && hdr[0] == kMagic && hdr[1] == 1 &&
+  uint32_t models = 1;
+  std::vector<float> mean(mlp::F), stdv(mlp::F), params;
+  bool ok = std::fread(hdr, sizeof(uint32_t), 4, fp) == 4 && hdr[0] == kMagic && (hdr[1] == 1 || hdr[1] == 2) &&
             (int)hdr[2] == mlp::F && (int)hdr[3] == mlp::H;
+  // Version 2 adds the number of networks after the header.
+  if (ok && hdr[1] == 2)
+    ok = std::fread(&models, sizeof(uint32_t), 1, fp) == 1 && models >= 1 && (int)models <= kSelMaxModels;
+  if (ok) params.resize((size_t)models * mlp::P);
   ok = ok && std::fread(mean.data(), sizeof(float), mlp::F, fp) == (size_t)mlp::F;
   ok = ok && std::fread(stdv.data(), sizeof(float), mlp::F, fp) == (size_t)mlp::F;
-  ok = ok && std::fread(params.data(), sizeof(float), mlp::P, fp) == (size_t)mlp::P;
+  ok = ok && std::fread(params.data(), sizeof(float), params.size(), fp) == params.size();
   std::fclose(fp);
   if (!ok) return LIBRETT_C_INVALID_PARAMETER;
-  return install_weights(params.data(), mlp::P, mean.data(), stdv.data(), mlp::F);
+  return install_weights(params.data(), (int)params.size(), mean.data(), stdv.data(), mlp::F);
 }
 
 // Directory of the shared library that holds this code.
@@ -190,6 +199,14 @@ uint64_t selector_generation() {
   return s.gen;
 }
 
+double selector_margin() {
+  static const double m = [] {
+    const char* v = std::getenv("LIBRETT_SELECTOR_MARGIN");
+    return (v && *v) ? std::atof(v) : kSelMargin;
+  }();
+  return m;
+}
+
 int selector_choose(const std::vector<Candidate>& cands, const Problem& full, int device) {
   load_from_env();
   Selector& s = state();
@@ -212,7 +229,9 @@ int selector_choose(const std::vector<Candidate>& cands, const Problem& full, in
   DevBuf& d = s.dev[device];
   bool ok = true;
   if (!d.stream) ok = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) == hipSuccess;
-  if (ok && !d.params) ok = hipMalloc((void**)&d.params, mlp::P * sizeof(float)) == hipSuccess;
+  const int models = (int)(s.params.size() / mlp::P);
+  if (ok && !d.params)
+    ok = hipMalloc((void**)&d.params, (size_t)kSelMaxModels * mlp::P * sizeof(float)) == hipSuccess;
   if (ok && d.cap < n) {
     if (d.x) (void)hipFree(d.x);
     if (d.out) (void)hipFree(d.out);
@@ -220,30 +239,40 @@ int selector_choose(const std::vector<Candidate>& cands, const Problem& full, in
     d.cap = 0;
     const int cap = (n + 63) / 64 * 64;
     ok = hipMalloc((void**)&d.x, (size_t)cap * mlp::F * sizeof(float)) == hipSuccess &&
-         hipMalloc((void**)&d.out, (size_t)cap * sizeof(float)) == hipSuccess;
+         hipMalloc((void**)&d.out, (size_t)kSelMaxModels * cap * sizeof(float)) == hipSuccess;
     if (ok) d.cap = cap;
   }
   if (ok && d.gen != s.gen) {
-    ok = hipMemcpyAsync(d.params, s.params.data(), mlp::P * sizeof(float), hipMemcpyHostToDevice, d.stream) ==
-         hipSuccess;
+    ok = hipMemcpyAsync(d.params, s.params.data(), s.params.size() * sizeof(float), hipMemcpyHostToDevice,
+                        d.stream) == hipSuccess;
     if (ok) d.gen = s.gen;
   }
-  std::vector<float> score(n);
+  // One launch scores every candidate with every network (network k writes
+  // out[k * n ..]); the score is their mean.
+  std::vector<float> all((size_t)models * n), score(n, 0.0f);
   ok = ok &&
        hipMemcpyAsync(d.x, x.data(), x.size() * sizeof(float), hipMemcpyHostToDevice, d.stream) == hipSuccess &&
-       mlp::launch_forward(d.params, d.x, n, d.out, d.stream) == hipSuccess &&
-       hipMemcpyAsync(score.data(), d.out, n * sizeof(float), hipMemcpyDeviceToHost, d.stream) == hipSuccess &&
+       mlp::launch_forward(d.params, d.x, n, d.out, d.stream, models) == hipSuccess &&
+       hipMemcpyAsync(all.data(), d.out, all.size() * sizeof(float), hipMemcpyDeviceToHost, d.stream) ==
+           hipSuccess &&
        hipStreamSynchronize(d.stream) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
     d.gen = 0;
     return -1;
   }
+  for (int k = 0; k < models; k++)
+    for (int i = 0; i < n; i++) score[i] += all[(size_t)k * n + i];
+  for (int i = 0; i < n; i++) score[i] /= (float)models;
   int best = -1;
   for (int i = 0; i < n; i++) {
     if (!std::isfinite(score[i])) return -1;
     if (best < 0 || score[i] < score[best]) best = i;
   }
+  // The analytic model's choice stands unless the selector predicts its own
+  // pick to be faster by more than the margin.
+  const int mc = choose_by_model(cands);
+  if (mc >= 0 && mc < n && score[mc] - score[best] < selector_margin()) return mc;
   return best;
 }
 
@@ -266,6 +295,8 @@ int librett_c_selector_set(const float* params, int nparams, const float* mean, 
 int librett_c_selector_load(const char* path) { return load_weights(path); }
 
 unsigned long long librett_c_selector_generation(void) { return selector_generation(); }
+
+double librett_c_selector_margin(void) { return selector_margin(); }
 
 int librett_c_mlp_forward(const float* params, const float* x, int n, float* out, void* stream) {
   if (!params || !x || !out || n < 0) return LIBRETT_C_INVALID_PARAMETER;

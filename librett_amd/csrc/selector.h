@@ -13,7 +13,10 @@ namespace lrt {
 
 namespace mlp {
 
-hipError_t launch_forward(const float* params, const float* xs, int n, float* out, hipStream_t stream);
+// out[k * n + i] = network k (parameters params + k * P) on sample i, for
+// k < models.
+hipError_t launch_forward(const float* params, const float* xs, int n, float* out, hipStream_t stream,
+                          int models = 1);
 hipError_t prepare_grad();
 hipError_t launch_grad(const float* params, const float* xs, const float* ys, const float* ws, const int* idx,
                        int base, int end, float inv_m, int nblocks, float* partial, float* loss, hipStream_t stream);
@@ -34,6 +37,8 @@ constexpr int kSelMaxRank = 7;
 
 // Features per (problem, candidate) pair (the MLP's input width).
 constexpr int kSelFeatures = 50;
+// Largest ensemble the selector averages.
+constexpr int kSelMaxModels = 16;
 
 // Raw (unnormalised) features of candidate c of problem `full`; `reduced`
 // is reduce_ranks(full). Writes mlp::F floats.
@@ -47,5 +52,13 @@ int selector_choose(const std::vector<Candidate>& cands, const Problem& full, in
 
 // Generation of the loaded weights (0: none); part of the plan-cache key.
 uint64_t selector_generation();
+
+// The selector overrides the analytic model's choice only when its scores
+// (predicted log2 time over the roofline time) put its own pick below the
+// analytic choice by more than this margin. 0.1 (about 7%) by default;
+// LIBRETT_SELECTOR_MARGIN sets another value (0: the selector's pick
+// always).
+constexpr double kSelMargin = 0.1;
+double selector_margin();
 
 }  // namespace lrt

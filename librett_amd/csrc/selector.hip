@@ -14,6 +14,9 @@ namespace mlp {
 __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ params, const float* __restrict__ xs,
                                                      int n, float* __restrict__ out) {
   __shared__ float p[P];
+  // blockIdx.y selects the network of an ensemble.
+  params += (long long)blockIdx.y * P;
+  out += (long long)blockIdx.y * n;
   for (int e = threadIdx.x; e < P; e += blockDim.x) p[e] = params[e];
   __syncthreads();
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -59,11 +62,13 @@ __global__ __launch_bounds__(256) void k_mlp_adam(const float* __restrict__ part
   if (e < P) adam_update(e, partial, nparts, params, m, v, lr, beta1, beta2, c1, c2, eps);
 }
 
-hipError_t launch_forward(const float* params, const float* xs, int n, float* out, hipStream_t stream) {
+hipError_t launch_forward(const float* params, const float* xs, int n, float* out, hipStream_t stream,
+                          int models) {
   if (n <= 0) return hipSuccess;
+  if (models < 1) return hipErrorInvalidValue;
   void* args[4] = {&params, &xs, &n, &out};
-  return hipLaunchKernel(reinterpret_cast<const void*>(&k_mlp_forward), dim3((n + 255) / 256), dim3(256), args, 0,
-                         stream);
+  return hipLaunchKernel(reinterpret_cast<const void*>(&k_mlp_forward), dim3((n + 255) / 256, models), dim3(256),
+                         args, 0, stream);
 }
 
 hipError_t prepare_grad() {

@@ -44,26 +44,37 @@ DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "sele
 # ---------------------------------------------------------------------------
 
 def save_weights(path, params, mean, std):
+    """One network (version 1) or an ensemble of k networks sharing the
+    normalisation (version 2: k after the header, then k parameter sets)."""
     from . import mlp_sizes
     nf, nh, npar = mlp_sizes()
-    params = np.asarray(params, dtype="<f4")
-    assert params.size == npar and len(mean) == nf and len(std) == nf
+    params = np.asarray(params, dtype="<f4").reshape(-1)
+    k = params.size // npar
+    assert k >= 1 and params.size == k * npar and len(mean) == nf and len(std) == nf
     with open(path, "wb") as fh:
-        fh.write(struct.pack("<4I", MAGIC, 1, nf, nh))
+        if k == 1:
+            fh.write(struct.pack("<4I", MAGIC, 1, nf, nh))
+        else:
+            fh.write(struct.pack("<5I", MAGIC, 2, nf, nh, k))
         fh.write(np.asarray(mean, dtype="<f4").tobytes())
         fh.write(np.asarray(std, dtype="<f4").tobytes())
         fh.write(params.tobytes())
 
 
 def read_weights(path):
+    """(params of all networks, flat; mean; std)."""
     from . import mlp_sizes
     nf, nh, npar = mlp_sizes()
     raw = open(path, "rb").read()
     magic, ver, f, h = struct.unpack("<4I", raw[:16])
-    if (magic, ver, f, h) != (MAGIC, 1, nf, nh):
+    if magic != MAGIC or ver not in (1, 2) or (f, h) != (nf, nh):
         raise ValueError("not a selector weights file: %s" % path)
-    a = np.frombuffer(raw[16:], dtype="<f4")
-    if a.size != 2 * nf + npar:
+    k, head = 1, 16
+    if ver == 2:
+        k, = struct.unpack("<I", raw[16:20])
+        head = 20
+    a = np.frombuffer(raw[head:], dtype="<f4")
+    if k < 1 or a.size != 2 * nf + k * npar:
         raise ValueError("truncated selector weights file: %s" % path)
     return a[2 * nf:].copy(), a[:nf].copy(), a[nf:2 * nf].copy()
 
@@ -248,19 +259,35 @@ def train(recs, stream, epochs=60, lr=2e-3, minibatch=2048, seed=1, log=None):
     return p, mean, std, np.array(losses)
 
 
+def train_ensemble(recs, stream, models=1, epochs=60, lr=2e-3, seed=1, log=None):
+    """Trains `models` networks with seeds seed, seed + 1, ... on the same
+    data; returns (all parameters, flat; mean; std; last network's losses)."""
+    ps = []
+    for k in range(models):
+        p, mean, std, losses = train(recs, stream, epochs, lr, seed=seed + k, log=log)
+        ps.append(p)
+    return np.concatenate(ps), mean, std, losses
+
+
 def scores(params, mean, std, recs, stream):
-    """MLP scores of every candidate, computed by the HIP inference kernel."""
-    from . import hip, mlp_forward
+    """MLP scores of every candidate, computed by the HIP inference kernel;
+    the mean over the networks of an ensemble."""
+    from . import hip, mlp_forward, mlp_sizes
+    npar = mlp_sizes()[2]
     x = np.concatenate([r["feats"] for r in recs]).astype(np.float32)
     xn = ((x - mean) / std).astype(np.float32)
-    out = np.zeros(len(xn), dtype=np.float32)
-    p = np.asarray(params, dtype=np.float32)
-    dx, dp, do = hip.DeviceBuffer(max(4, xn.nbytes)), hip.DeviceBuffer(p.nbytes), hip.DeviceBuffer(max(4, out.nbytes))
+    nets = np.asarray(params, dtype=np.float32).reshape(-1, npar)
+    out = np.zeros(len(xn), dtype=np.float64)
+    one = np.zeros(len(xn), dtype=np.float32)
+    dx, dp, do = hip.DeviceBuffer(max(4, xn.nbytes)), hip.DeviceBuffer(npar * 4), hip.DeviceBuffer(max(4, one.nbytes))
     hip.htod(dx.ptr, xn)
-    hip.htod(dp.ptr, p)
-    mlp_forward(dp.ptr, dx.ptr, len(xn), do.ptr, stream)
-    hip.stream_sync(stream)
-    hip.dtoh(out, do.ptr)
+    for p in nets:
+        hip.htod(dp.ptr, np.ascontiguousarray(p))
+        mlp_forward(dp.ptr, dx.ptr, len(xn), do.ptr, stream)
+        hip.stream_sync(stream)
+        hip.dtoh(one, do.ptr)
+        out += one
+    out = (out / len(nets)).astype(np.float32)
     for b in (dx, dp, do):
         b.free()
     res, o = [], 0
@@ -270,9 +297,11 @@ def scores(params, mean, std, recs, stream):
     return res
 
 
-def evaluate(recs, sc):
+def evaluate(recs, sc, margin=0.0):
     """Slowdown of the selector's and the analytic model's choice versus the
-    fastest measured candidate, per problem."""
+    fastest measured candidate, per problem. With a margin, the selector's
+    pick replaces the analytic choice only when its score is lower by more
+    than the margin (librettPlan's rule)."""
     sel, ana = [], []
     for r, s in zip(recs, sc):
         t = np.asarray(r["times"], dtype=np.float64)
@@ -280,10 +309,12 @@ def evaluate(recs, sc):
         if not ok.any():
             continue
         best = t[ok].min()
-        s = np.where(ok, s, np.inf)
+        s = np.where(ok, np.asarray(s, dtype=np.float64), np.inf)
         i = int(np.argmin(s))
-        sel.append(t[i] / best)
         m = r["model_choice"]
+        if margin > 0 and 0 <= m < len(t) and s[m] - s[i] < margin:
+            i = m
+        sel.append(t[i] / best)
         ana.append(t[m] / best if 0 <= m < len(t) and t[m] > 0 else np.nan)
     return np.array(sel), np.array(ana)
 
@@ -320,6 +351,8 @@ def main(argv=None):
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--models", type=int, default=1,
+                    help="build / train: networks in the ensemble (seeds --seed, --seed + 1, ...)")
     a = ap.parse_args(argv)
     from . import hip
 
@@ -332,7 +365,7 @@ def main(argv=None):
         t0 = time.time()
         tr = collect(problems(a.train, a.seed), stream, A.ptr, B.ptr, log)
         t1 = time.time()
-        p, mean, std, losses = train(tr, stream, a.epochs, a.lr, log=log)
+        p, mean, std, losses = train_ensemble(tr, stream, a.models, a.epochs, a.lr, a.seed, log)
         t2 = time.time()
         save_weights(a.weights, p, mean, std)
         ho = collect(problems(a.heldout, a.seed + 1000), stream, A.ptr, B.ptr, log)
@@ -361,12 +394,13 @@ def main(argv=None):
         if any(r["feats"] is None for r in recs):
             recs = attach_features(recs, log)
         t1 = time.time()
-        p, mean, std, losses = train(recs, stream, a.epochs, a.lr, log=log)
+        p, mean, std, losses = train_ensemble(recs, stream, a.models, a.epochs, a.lr, a.seed, log)
         t2 = time.time()
         save_weights(a.weights, p, mean, std)
         s_tr, a_tr = evaluate(recs, scores(p, mean, std, recs, stream))
         rep = {"train_problems": len(recs), "samples": int(sum(len(r["times"]) for r in recs)),
                "epochs": a.epochs, "lr": a.lr, "final_mse": float(losses[-1]), "train_s": t2 - t1, "seed": a.seed,
+               "models": a.models,
                "train": {"selector": summary(s_tr), "analytic": summary(a_tr)}}
         if a.heldout > 0:  # --heldout 0 leaves the held-out evaluation to a run of its own
             A, B = _buffers(512 << 20)

@@ -152,6 +152,10 @@ def test_plan_follows_selector(gpu_stream, dim, perm, elem):
     assert L.selector_set(p, np.zeros(M.F), np.ones(M.F)) == L.LIBRETT_SUCCESS
     try:
         want = int(np.argmin(feats[:, k]))
+        # librettPlan keeps the analytic choice unless the selector's score
+        # for it is higher than its own pick's by more than the margin.
+        if feats[model_choice, k] - feats[want, k] < L.selector_margin():
+            want = model_choice
         got = L.selector_choose(dim, perm, elem)
         assert got == want, (got, want, k)
         r, h = L.librettPlan(len(dim), dim, perm, elem, gpu_stream)
@@ -176,3 +180,37 @@ def test_plan_follows_selector(gpu_stream, dim, perm, elem):
     r, h = L.librettPlan(len(dim), dim, perm, elem, gpu_stream)
     assert L.plan_info(h)["desc"] == L.candidates(dim, perm, elem)[model_choice]["desc"]
     L.librettDestroy(h)
+
+
+def test_plan_follows_ensemble_mean(gpu_stream):
+    dim, perm, elem = [2, 8, 16, 32, 64, 8], [5, 3, 1, 0, 2, 4], 4
+    say("case: librettPlan with a two-network selector", dim, perm, elem)
+    feats, model_choice = L.selector_features(dim, perm, elem)
+    order = np.argsort(-feats.std(0))
+    j, k = int(order[0]), int(order[1])
+
+    def net(f, scale):
+        W1 = np.zeros((M.H, M.F)); W1[0, f] = scale
+        b1 = np.zeros(M.H); b1[0] = 1000
+        W2 = np.zeros((M.H, M.H)); W2[0, 0] = 1
+        w3 = np.zeros(M.H); w3[0] = 1
+        return M.pack(W1, b1, W2, np.zeros(M.H), w3, 0.0)
+
+    # Scores 3 f_j + 1000 and f_k + 1000: the ensemble ranks by 1.5 f_j + 0.5 f_k.
+    p = np.concatenate([net(j, 3.0), net(k, 1.0)]).astype(np.float32)
+    assert L.selector_set(p, np.zeros(M.F), np.ones(M.F)) == L.LIBRETT_SUCCESS
+    try:
+        score = 1.5 * feats[:, j].astype(np.float64) + 0.5 * feats[:, k]
+        margin, low = L.selector_margin(), float(score.min())
+        got = L.selector_choose(dim, perm, elem)
+        # float32 scores around 1000: allow 1e-3 for rounding at ties.
+        if score[model_choice] - low < margin - 1e-3:
+            assert got == model_choice
+        elif score[model_choice] - low > margin + 1e-3:
+            assert score[got] <= low + 1e-3
+        r, h = L.librettPlan(len(dim), dim, perm, elem, gpu_stream)
+        assert r == L.LIBRETT_SUCCESS
+        assert L.plan_info(h)["desc"] == L.candidates(dim, perm, elem)[got]["desc"]
+        L.librettDestroy(h)
+    finally:
+        L.selector_set(None, None, None)

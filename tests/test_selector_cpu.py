@@ -2,6 +2,10 @@
 pinned by a hand-worked network, finite differences and Adam's closed-form
 first step; the HIP kernels' per-thread code (host replay) against it; the
 features; the weights file."""
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 
@@ -167,6 +171,28 @@ def test_weights_file_roundtrip(tmp_path):
     assert L.selector_generation() == 0
 
 
+def test_ensemble_weights_file(tmp_path):
+    p = np.concatenate([rand_params(k) for k in (21, 22, 23)])
+    mean = np.linspace(-1, 1, M.F).astype(np.float32)
+    std = np.linspace(0.5, 2, M.F).astype(np.float32)
+    path = str(tmp_path / "ens.bin")
+    S.save_weights(path, p, mean, std)
+    raw = open(path, "rb").read()
+    assert raw[4:8] == (2).to_bytes(4, "little") and raw[16:20] == (3).to_bytes(4, "little")
+    assert len(raw) == 20 + 4 * (2 * M.F + 3 * M.P)
+    p2, m2, s2 = S.read_weights(path)
+    assert np.array_equal(p, p2) and np.array_equal(mean, m2) and np.array_equal(std, s2)
+    g0 = L.selector_generation()
+    assert L.selector_load(path) == L.LIBRETT_SUCCESS and L.selector_generation() > g0
+    # A parameter count that is not a whole number of networks, or more
+    # networks than the selector averages, is refused.
+    assert L.selector_set(p[:M.P + 1], mean, std) == L.LIBRETT_INVALID_PARAMETER
+    assert L.selector_set(np.tile(p, 6), mean, std) == L.LIBRETT_INVALID_PARAMETER
+    open(str(tmp_path / "short.bin"), "wb").write(raw[:-4])
+    assert L.selector_load(str(tmp_path / "short.bin")) == L.LIBRETT_INVALID_PARAMETER
+    assert L.selector_set(None, None, None) == L.LIBRETT_SUCCESS
+
+
 def test_problem_generator_ranges():
     for d, p, e in S.problems(300, 5):
         assert 2 <= len(d) <= 7 and sorted(p) == list(range(len(d))) and e in (4, 8, 16)
@@ -201,3 +227,28 @@ def test_unloading_first_keeps_analytic_model(tmp_path):
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert r.stdout.strip() == "0"
+
+
+def test_margin_rule_in_evaluation():
+    # Two candidates: the analytic model picks 1 (20% slower than 0).
+    rec = {"times": np.array([1.0, 1.2]), "model_choice": 1}
+    close, far = np.array([0.0, 0.05]), np.array([0.0, 0.2])
+    sel, ana = S.evaluate([rec, rec], [close, far], margin=0.1)
+    assert list(sel) == [1.2, 1.0] and list(ana) == [1.2, 1.2]
+    sel, _ = S.evaluate([rec, rec], [close, far])
+    assert list(sel) == [1.0, 1.0]
+    # A candidate that did not run is never chosen.
+    rec2 = {"times": np.array([-1.0, 1.0, 2.0]), "model_choice": 2}
+    sel, _ = S.evaluate([rec2], [np.array([-5.0, 0.0, 0.5])], margin=0.1)
+    assert list(sel) == [1.0]
+
+
+def test_margin_default_and_override():
+    code = "import librett_amd as L; print(L.selector_margin())"
+    env = dict(os.environ)
+    env.pop("LIBRETT_SELECTOR_MARGIN", None)
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, check=True)
+    assert float(out.stdout) == 0.1
+    env["LIBRETT_SELECTOR_MARGIN"] = "0"
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, check=True)
+    assert float(out.stdout) == 0.0
