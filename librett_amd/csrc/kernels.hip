@@ -236,6 +236,10 @@ struct PackedEntry {
 #define LRT_TILED_V(B, TX, TY, V) \
   LRT_TILED(B, TX, TY, 1, 1) LRT_TILED(B, TX, TY, V, 1) LRT_TILED(B, TX, TY, 1, V) LRT_TILED(B, TX, TY, V, V)
 
+#define LRT_TILED_2(TX, TY) \
+  LRT_TILED(4, TX, TY, 2, 1) LRT_TILED(4, TX, TY, 1, 2) LRT_TILED(4, TX, TY, 2, 2) LRT_TILED(4, TX, TY, 4, 2) \
+  LRT_TILED(4, TX, TY, 2, 4)
+
 static const TiledEntry kTiledTable[] = {
     LRT_TILED_V(4, 64, 64, 4) LRT_TILED_V(4, 32, 32, 4) LRT_TILED_V(4, 64, 32, 4) LRT_TILED_V(4, 32, 64, 4)
     LRT_TILED_V(8, 32, 32, 2) LRT_TILED_V(8, 64, 64, 2) LRT_TILED_V(8, 64, 32, 2) LRT_TILED_V(8, 32, 64, 2)
@@ -243,6 +247,10 @@ static const TiledEntry kTiledTable[] = {
     LRT_TILED_V(4, 128, 128, 4) LRT_TILED_V(4, 128, 64, 4) LRT_TILED_V(4, 64, 128, 4)
     LRT_TILED_V(8, 128, 64, 2) LRT_TILED_V(8, 64, 128, 2)
     LRT_TILED(16, 16, 16, 1, 1) LRT_TILED(16, 32, 32, 1, 1) LRT_TILED(16, 32, 16, 1, 1) LRT_TILED(16, 16, 32, 1, 1)
+    // f32 with 8-byte lanes on a side whose extent is even but not a
+    // multiple of 4 (half the wave instructions of 4-byte lanes).
+    LRT_TILED_2(64, 64) LRT_TILED_2(32, 32) LRT_TILED_2(64, 32) LRT_TILED_2(32, 64) LRT_TILED_2(128, 64)
+    LRT_TILED_2(64, 128)
 };
 
 #define LRT_COPY(W, TX) \

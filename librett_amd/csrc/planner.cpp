@@ -280,9 +280,7 @@ void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& 
   const int B = r.elem;
   const int ydim = r.perm[0];
   const int X = r.dim[0], Y = r.dim[ydim];
-  const int v = 16 / B;
-  const int vx = (v > 1 && X % v == 0) ? v : 1;
-  const int vy = (v > 1 && Y % v == 0) ? v : 1;
+  const int vmax = 16 / B;  // lanes move at most 16 bytes
   // Tile shapes (x, y). 4- and 8-byte elements also get tiles whose rows
   // are 512 bytes long in one or both directions: on MI355X they move 3-18%
   // faster than 64x64 where both dims allow 16-byte vectors (f32), less for
@@ -305,11 +303,24 @@ void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& 
     c.source = "reduced";
     c.tile_x = shapes[s][0];
     c.tile_y = shapes[s][1];
+    // Widest lanes (vx elements per input access, vy per output access)
+    // that divide the plane's extents and have a kernel instantiated;
+    // ties go to the larger sum.
+    int vx = 0, vy = 0;
+    for (int ax : {4, 2, 1})
+      for (int ay : {4, 2, 1}) {
+        if (ax > vmax || ay > vmax || X % ax || Y % ay) continue;
+        if (dev.batch_mode ? !batch_tiled_available(B, c.tile_x, c.tile_y, ax, ay)
+                           : !tiled_shape_available(B, c.tile_x, c.tile_y, ax, ay))
+          continue;
+        if (ax * ay > vx * vy || (ax * ay == vx * vy && ax + ay > vx + vy)) {
+          vx = ax;
+          vy = ay;
+        }
+      }
+    if (vx == 0) continue;
     c.vec_x = vx;
     c.vec_y = vy;
-    if (dev.batch_mode ? !batch_tiled_available(B, c.tile_x, c.tile_y, vx, vy)
-                       : !tiled_shape_available(B, c.tile_x, c.tile_y, vx, vy))
-      continue;
     // Skip tiles more than twice the plane in a direction when a smaller one exists.
     TiledArgs& a = c.tiled;
     a.X = X;

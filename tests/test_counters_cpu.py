@@ -30,6 +30,7 @@ PROBLEMS = [
     ((2, 3, 4, 5, 6), (4, 2, 0, 3, 1), 4),
     ((40, 3, 50), (0, 2, 1), 16),
     ((256, 8, 9), (0, 2, 1), 8),
+    ((34, 18, 3), (1, 0, 2), 4),      # 8-byte f32 lanes
 ]
 
 

@@ -106,6 +106,9 @@ def test_every_candidate_small_shapes(gpu_stream, bufs, elem):
         ([2, 8, 16, 32, 6], [4, 3, 1, 0, 2]),
         ([24, 32, 16, 3, 4, 9], [5, 1, 4, 2, 3, 0]),
         ([7, 1, 300], [2, 1, 0]),
+        ([34, 18, 5], [1, 0, 2]),   # f32: 8-byte lanes both ways
+        ([130, 46], [1, 0]),        # f32: 8-byte lanes, partial tiles
+        ([68, 50, 3], [1, 0, 2]),   # f32: 16-byte input lanes, 8-byte output lanes
     ]
     for dim, perm in shapes:
         n = len(L.candidates(dim, perm, elem))
@@ -167,7 +170,8 @@ def test_plan_measure(gpu_stream, bufs):
 
 def test_unaligned_pointers_use_fallback(gpu_stream, bufs):
     # Offsets of one element defeat 16-byte vector accesses.
-    for dim, perm, elem in [([256, 128], [1, 0], 4), ([64, 32, 8], [0, 2, 1], 4), ([128, 96], [1, 0], 8)]:
+    for dim, perm, elem in [([256, 128], [1, 0], 4), ([64, 32, 8], [0, 2, 1], 4), ([128, 96], [1, 0], 8),
+                            ([130, 46], [1, 0], 4)]:
         for off in [(elem, 0), (0, elem), (elem, elem)]:
             say("case: unaligned", dim, perm, elem, off)
             check(dim, perm, elem, gpu_stream, bufs, offset=off)

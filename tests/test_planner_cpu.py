@@ -95,6 +95,19 @@ def test_hand_picked_shapes(elem):
         replay_all(dim, perm, elem)
 
 
+def test_f32_eight_byte_lanes():
+    # Plane extents that are even but not multiples of 4 get 8-byte lanes on
+    # that side (4 and 2, 2 and 4, 2 and 2, 2 and 1, 1 and 2), full and
+    # partial tiles.
+    seen = set()
+    for dim, perm in (([34, 18], [1, 0]), ([68, 50, 3], [1, 0, 2]), ([30, 36], [1, 0]), ([130, 46], [1, 0]),
+                      ([6, 7, 10], [2, 1, 0]), ([9, 70], [1, 0]), ([70, 9], [1, 0])):
+        for c in replay_all(dim, perm, 4):
+            if c["method"] == 4:
+                seen.add((c["vec_x"], c["vec_y"]))
+    assert {(2, 2), (4, 2), (2, 4), (2, 1), (1, 2)} <= seen
+
+
 def test_bench7_high_rank_scaled():
     # tests/librett_bench.cpp:718,738 shapes scaled down for the host replay.
     replay_all([5, 3, 2, 4, 3, 3, 3, 4], [7, 2, 5, 0, 1, 3, 6, 4], 8)
