@@ -564,6 +564,18 @@ def bench_serving(args, stream):
     hip.stream_sync(stream)
     loop_gbs = moved / (time.perf_counter() - t0) / 1e9
     log("per-call loop: %.1f GB/s" % loop_gbs)
+    # The same calls with a stream synchronisation after each one: a request
+    # handler without batching that returns every result before taking the
+    # next request.
+    t0 = time.perf_counter()
+    moved = 0
+    for i in range(args.requests):
+        h, nb = handles[picks[i]]
+        L.librettExecute(h, pool.ptr + int(offs[i]), out.ptr)
+        hip.stream_sync(stream)
+        moved += 2 * nb
+    sync_gbs = moved / (time.perf_counter() - t0) / 1e9
+    log("per-call loop, synchronised per request: %.1f GB/s" % sync_gbs)
     for h, _ in handles:
         L.librettDestroy(h)
     out.free()
@@ -590,6 +602,8 @@ def bench_serving(args, stream):
         "load_sweep": rates,
         "per_call_loop_gbs": round(loop_gbs, 1),
         "speedup_vs_per_call": round(sat["gbs"] / loop_gbs, 2),
+        "per_call_synchronised_gbs": round(sync_gbs, 1),
+        "speedup_vs_per_call_synchronised": round(sat["gbs"] / sync_gbs, 2),
     }
     print(json.dumps(line), flush=True)
 
