@@ -308,10 +308,19 @@ struct PackedOps {
   __host__ __device__ static void load(const Pre& p, int off_in, int t, int nt,
                                        const E* __restrict__ in, E* lds) {
     E r[NREG];
+    fetch(p, off_in, t, nt, in, r);
+    stash(p, t, nt, r, lds);
+  }
+
+  // The two halves of load(): global memory to registers, registers to LDS.
+  __host__ __device__ static void fetch(const Pre& p, int off_in, int t, int nt, const E* __restrict__ in,
+                                        E* r) {
     const E* base = in + (p.base_in + off_in);
 #pragma unroll
     for (int j = 0; j < NREG; j++)
       if (t + j * nt < p.vol) r[j] = ld(base + p.in[j]);
+  }
+  __host__ __device__ static void stash(const Pre& p, int t, int nt, const E* r, E* lds) {
 #pragma unroll
     for (int j = 0; j < NREG; j++)
       if (t + j * nt < p.vol) {

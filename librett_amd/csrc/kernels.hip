@@ -12,6 +12,24 @@
 #include "kernel_ops.h"
 #include "kernels.h"
 
+// k_packed with 12 or more elements per thread fetches the next round's
+// input into registers while it stores the current round from LDS. Such
+// workgroups are large and few per CU, and without the overlap each round
+// waits out its full load latency: on MI355X the pipelined form ran these
+// plans up to 1.8x faster, while for 2-8 elements per thread it was within
+// a few percent either way (slower on the bench's Packed cases), so those
+// fetch each round after the previous one is stored. For 8-byte elements
+// with 16 per thread the extra registers no longer fit the 128 VGPRs of a
+// 1024-thread workgroup and spill, so that instantiation is not pipelined.
+// LRT_PACKED_PIPE=0 turns the pipeline off everywhere.
+#ifndef LRT_PACKED_PIPE
+#define LRT_PACKED_PIPE 1
+#endif
+template <int B, int NREG>
+constexpr bool packed_pipelined() {
+  return LRT_PACKED_PIPE && NREG >= 12 && !(B == 8 && NREG == 16);
+}
+
 namespace lrt {
 
 // ---------------------------------------------------------------------------
@@ -75,13 +93,37 @@ __global__ __launch_bounds__(1024) void k_packed(PackedArgs a, const void* __res
     step = gridDim.y;
   }
   Ops::prepare(a, split, t, nt, p);
-  for (int o = o0; o < o1; o += step) {
-    int off_in, off_out;
-    outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
-    Ops::load(p, off_in, t, nt, src, lds);
-    __syncthreads();
-    Ops::store(p, off_out, t, nt, dst, lds);
-    __syncthreads();
+  if constexpr (packed_pipelined<B, NREG>()) {
+    // Round o is stashed from registers into LDS; while it is stored from
+    // LDS, round o + step is fetched into the same registers.
+    E r[NREG];
+    if (o0 < o1) {
+      int off_in, off_out;
+      outer_offsets(a.outer, (uint32_t)o0, off_in, off_out);
+      Ops::fetch(p, off_in, t, nt, src, r);
+    }
+    for (int o = o0; o < o1; o += step) {
+      int off_in, off_out;
+      outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
+      Ops::stash(p, t, nt, r, lds);
+      __syncthreads();
+      if (o + step < o1) {
+        int next_in, next_out;
+        outer_offsets(a.outer, (uint32_t)(o + step), next_in, next_out);
+        Ops::fetch(p, next_in, t, nt, src, r);
+      }
+      Ops::store(p, off_out, t, nt, dst, lds);
+      __syncthreads();
+    }
+  } else {
+    for (int o = o0; o < o1; o += step) {
+      int off_in, off_out;
+      outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
+      Ops::load(p, off_in, t, nt, src, lds);
+      __syncthreads();
+      Ops::store(p, off_out, t, nt, dst, lds);
+      __syncthreads();
+    }
   }
 }
 
@@ -195,17 +237,51 @@ static void replay_packed(const Candidate& c, const void* in, void* out) {
       step = c.grid_y;
     }
     for (int t = 0; t < nt; t++) Ops::prepare(a, split, t, nt, pre[t]);
-    for (int o = o0; o < o1; o += step) {
-      int off_in, off_out;
-      outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
-      const long long item = (long long)split * a.vol_outer + o;
-      for (int t = 0; t < nt; t++) {
-        replay_at(item, t);
-        Ops::load(pre[t], off_in, t, nt, static_cast<const E*>(in), lds.data());
+    const E* src = static_cast<const E*>(in);
+    E* dst = static_cast<E*>(out);
+    if constexpr (packed_pipelined<B, NREG>()) {
+      // The device's schedule: fetch round o + step between the barrier
+      // after round o's stash and round o's store.
+      std::vector<E> regs((size_t)nt * NREG);
+      if (o0 < o1) {
+        int off_in, off_out;
+        outer_offsets(a.outer, (uint32_t)o0, off_in, off_out);
+        for (int t = 0; t < nt; t++) {
+          replay_at((long long)split * a.vol_outer + o0, t);
+          Ops::fetch(pre[t], off_in, t, nt, src, &regs[(size_t)t * NREG]);
+        }
       }
-      for (int t = 0; t < nt; t++) {
-        replay_at(item, t);
-        Ops::store(pre[t], off_out, t, nt, static_cast<E*>(out), lds.data());
+      for (int o = o0; o < o1; o += step) {
+        int off_in, off_out;
+        outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
+        const long long item = (long long)split * a.vol_outer + o;
+        for (int t = 0; t < nt; t++) Ops::stash(pre[t], t, nt, &regs[(size_t)t * NREG], lds.data());
+        if (o + step < o1) {
+          int next_in, next_out;
+          outer_offsets(a.outer, (uint32_t)(o + step), next_in, next_out);
+          for (int t = 0; t < nt; t++) {
+            replay_at(item + step, t);
+            Ops::fetch(pre[t], next_in, t, nt, src, &regs[(size_t)t * NREG]);
+          }
+        }
+        for (int t = 0; t < nt; t++) {
+          replay_at(item, t);
+          Ops::store(pre[t], off_out, t, nt, dst, lds.data());
+        }
+      }
+    } else {
+      for (int o = o0; o < o1; o += step) {
+        int off_in, off_out;
+        outer_offsets(a.outer, (uint32_t)o, off_in, off_out);
+        const long long item = (long long)split * a.vol_outer + o;
+        for (int t = 0; t < nt; t++) {
+          replay_at(item, t);
+          Ops::load(pre[t], off_in, t, nt, src, lds.data());
+        }
+        for (int t = 0; t < nt; t++) {
+          replay_at(item, t);
+          Ops::store(pre[t], off_out, t, nt, dst, lds.data());
+        }
       }
     }
   }

@@ -31,6 +31,7 @@ PROBLEMS = [
     ((40, 3, 50), (0, 2, 1), 16),
     ((256, 8, 9), (0, 2, 1), 8),
     ((34, 18, 3), (1, 0, 2), 4),      # 8-byte f32 lanes
+    ((20, 31, 17, 5), (3, 1, 0, 2), 4),  # Packed with 12 and 16 elements per thread (pipelined rounds)
 ]
 
 
