@@ -102,3 +102,39 @@ def test_full_size_seeded_problems(gpu_stream):
             for cls in ("keep", "move", "small"):
                 dim, perm = W.large_case(20261202 + seed, elem, cls)
                 check_full_size(dim, perm, elem, gpu_stream)
+
+
+def test_plan_with_exact_counts_is_correct(gpu_stream):
+    # LIBRETT_EXACT_COUNTS=1 makes librettPlan count every sampled Tiled /
+    # TiledCopy candidate on the device before choosing; the plan it picks
+    # must still transpose bit-exactly. Run in a fresh process (the variable
+    # is read once).
+    import os
+    import subprocess
+    import sys
+    say("case: librettPlan with LIBRETT_EXACT_COUNTS=1, (300, 200, 9)/(1, 2, 0) f32 and (1300, 700)/(1, 0) f64")
+    code = r'''
+import numpy as np, librett_amd as L, oracle
+from librett_amd import data, hip
+s = hip.stream_create()
+for dim, perm, elem in (([300, 200, 9], [1, 2, 0], 4), ([1300, 700], [1, 0], 8)):
+    n = int(np.prod(dim))
+    x = data.uniform(n, elem, 5)
+    a, b = hip.DeviceBuffer(n * elem), hip.DeviceBuffer(n * elem)
+    hip.htod(a.ptr, x)
+    r, h = L.librettPlan(len(dim), dim, perm, elem, s)
+    assert r == 0
+    assert L.librettExecute(h, a.ptr, b.ptr) == 0
+    hip.stream_sync(s)
+    y = np.empty_like(x)
+    hip.dtoh(y, b.ptr)
+    assert np.array_equal(y.view(np.uint8), oracle.transpose(dim, perm, elem, x).view(np.uint8))
+    print(L.plan_info(h)["desc"], flush=True)
+    L.librettDestroy(h)
+print("ok")
+'''
+    env = dict(os.environ, LIBRETT_EXACT_COUNTS="1")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, "-c", code], env=env, cwd=root, capture_output=True, text=True, timeout=120)
+    say(out.stdout, out.stderr[-2000:])
+    assert out.returncode == 0 and out.stdout.strip().endswith("ok")
