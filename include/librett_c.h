@@ -318,7 +318,11 @@ int librett_c_selector_features(int rank, const int* dim, const int* permutation
                                 const librett_c_device* device, float* feats, int max, int* count,
                                 int* model_choice);
 
-/* Candidate the loaded selector picks on the current GPU (-1: analytic). */
+/* Candidate librettPlan's selector step picks on the current GPU: the
+ * ensemble's argmin, or the analytic model's choice when the ensemble
+ * scores that within librett_c_selector_margin() of its own pick; *index
+ * is -1 when no weights are loaded or the problem is outside the
+ * selector's domain (librettPlan then uses the analytic model). */
 int librett_c_selector_choose(int rank, const int* dim, const int* permutation, size_t sizeofType, int* index);
 
 /* Loads weights: nparams floats in the layout W1[64][50], b1[64], W2[64][64],
