@@ -12,13 +12,15 @@
  * Elements are opaque 4-, 8- or 16-byte words and are copied bit for bit.
  *
  * This build is single-vendor: it always targets HIP on gfx950, so
- * LIBRETT_USES_HIP is defined here for code that tests it.
+ * LIBRETT_USES_HIP is defined here for code that tests it. Like the
+ * reference (src/librett.h:31-33) the header includes <hip/hip_runtime.h>,
+ * so callers that rely on that include keep compiling.
  */
 #ifndef LIBRETT_H
 #define LIBRETT_H
 
 #include <stddef.h>
-#include <hip/hip_runtime_api.h>
+#include <hip/hip_runtime.h>
 
 #ifndef LIBRETT_USES_HIP
 #define LIBRETT_USES_HIP 1

@@ -398,7 +398,9 @@ int librett_c_host_check_transpose(const void* odata, int rank, const int* dim, 
  * from `alloc` and go back through `free_fn` when installed (both NULL:
  * hipMalloc / hipFree). A buffer always returns to the allocator that made
  * it. Plans own no device memory. The selector's small per-device inference
- * buffers, kept for the life of the process, stay on hipMalloc.
+ * buffers are kept between plans; installing another allocator (or NULL)
+ * first returns them to the allocator that made them, and the next plan
+ * takes new ones from the allocator then installed.
  * allocator_live counts buffers currently held from a caller's allocator.
  * ------------------------------------------------------------------------ */
 typedef void* (*librett_c_alloc_fn)(size_t bytes, void* ctx);

@@ -17,7 +17,7 @@ if(NOT TARGET librett::librett)
     IMPORTED_SONAME "librett.so"
     INTERFACE_INCLUDE_DIRECTORIES "${_librett_prefix}/include;${_librett_prefix}/include/librett"
     INTERFACE_COMPILE_DEFINITIONS "LIBRETT_USES_HIP")
-  # The header includes <hip/hip_runtime_api.h>.
+  # The header includes <hip/hip_runtime.h>.
   if(NOT DEFINED ROCM_PATH)
     if(DEFINED ENV{ROCM_PATH})
       set(ROCM_PATH "$ENV{ROCM_PATH}")

@@ -5,6 +5,7 @@
 #include <unordered_map>
 
 #include "../../include/librett_c.h"
+#include "selector.h"
 
 namespace lrt {
 namespace {
@@ -79,10 +80,14 @@ extern "C" {
 int librett_c_set_allocator(librett_c_alloc_fn alloc, librett_c_free_fn free_fn, void* ctx) {
   if ((alloc == nullptr) != (free_fn == nullptr)) return LIBRETT_C_INVALID_PARAMETER;
   lrt::Allocator& A = lrt::alloc_state();
-  std::lock_guard<std::mutex> lock(A.mu);
-  A.alloc_fn = alloc;
-  A.free_fn = free_fn;
-  A.ctx = ctx;
+  // The selector's inference buffers go back to the allocator that made
+  // them before another one takes over.
+  lrt::selector_release_buffers([&] {
+    std::lock_guard<std::mutex> lock(A.mu);
+    A.alloc_fn = alloc;
+    A.free_fn = free_fn;
+    A.ctx = ctx;
+  });
   return LIBRETT_C_SUCCESS;
 }
 

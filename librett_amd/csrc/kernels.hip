@@ -7,6 +7,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 #include "kernel_ops.h"
@@ -460,22 +463,84 @@ bool replay_candidate(const Candidate& c, const void* in, void* out) {
   }
 }
 
+// The dynamic-LDS limit of a kernel is a process-wide attribute of that
+// kernel, shared by every plan that uses it. It is only ever raised, and
+// raised straight to the device's per-workgroup maximum when the runtime
+// accepts that, so no plan can lower the limit another plan's launches rely
+// on, and planning threads cannot race it downward. Launches above the
+// default limit need it.
+static std::mutex g_lds_mu;
+static std::map<std::pair<int, const void*>, int> g_lds_limit;
+
+static bool ensure_lds_limit(const void* k, int bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  std::lock_guard<std::mutex> lock(g_lds_mu);
+  int& cur = g_lds_limit[{dev, k}];
+  if (cur >= bytes) return true;
+  int maxb = 0;
+  if (hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    maxb = 0;
+  }
+  if (maxb >= bytes && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, maxb) == hipSuccess) {
+    cur = maxb;
+    return true;
+  }
+  (void)hipGetLastError();
+  if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess) {
+    cur = bytes;
+    return true;
+  }
+  (void)hipGetLastError();
+  return false;
+}
+
+// Occupancy answers per (device, kernel, block, dynamic LDS): planning asks
+// for the same few combinations over and over (the reference keeps the same
+// kind of cache, nabCache, src/kernel.cpp:671-772; its key confuses 8- and
+// 16-byte kernels, this one names the kernel itself).
+struct OccKey {
+  int dev;
+  const void* k;
+  int block, lds;
+  bool operator<(const OccKey& o) const {
+    if (dev != o.dev) return dev < o.dev;
+    if (k != o.k) return k < o.k;
+    if (block != o.block) return block < o.block;
+    return lds < o.lds;
+  }
+};
+static std::mutex g_occ_mu;
+static std::map<OccKey, int> g_occ;
+
 int query_occupancy(const Candidate& c) {
   const void* k = kernel_of(c);
   if (!k) return 0;
-  if (c.method == kPacked || c.method == kPackedSplit) {
-    // Dynamic LDS above the default per-launch limit needs the attribute.
-    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      return 0;
-    }
-  }
-  int n = 0;
-  const size_t lds = (c.method == kPacked || c.method == kPackedSplit) ? c.lds_bytes : 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, c.block, lds) != hipSuccess) {
+  const bool dyn = c.method == kPacked || c.method == kPackedSplit;
+  const int lds = dyn ? c.lds_bytes : 0;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
     (void)hipGetLastError();
     return 0;
   }
+  const OccKey key{dev, k, c.block, lds};
+  {
+    std::lock_guard<std::mutex> lock(g_occ_mu);
+    auto it = g_occ.find(key);
+    if (it != g_occ.end()) return it->second;
+  }
+  if (dyn && !ensure_lds_limit(k, lds)) return 0;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, c.block, (size_t)lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  std::lock_guard<std::mutex> lock(g_occ_mu);
+  g_occ[key] = n;
   return n;
 }
 
@@ -483,7 +548,7 @@ int prepare_kernel(const Candidate& c) {
   if (c.method != kPacked && c.method != kPackedSplit) return 0;
   const void* k = kernel_of(c);
   if (!k) return -1;
-  return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes) == hipSuccess ? 0 : -1;
+  return ensure_lds_limit(k, c.lds_bytes) ? 0 : -1;
 }
 
 }  // namespace lrt

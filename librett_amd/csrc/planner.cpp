@@ -2,10 +2,15 @@
 #include "planner.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <mutex>
+#include <thread>
 
 #include "kernels.h"
 
@@ -85,6 +90,88 @@ void build_outer(const Problem& p, const bool* skip, const long long* cin, const
     m.ct_in[r] = 0;
     m.ct_out[r] = 0;
   }
+}
+
+// Planning threads. Candidates of one problem are modelled in parallel by a
+// small pool of worker threads that lives for the whole process (created on
+// first use, never joined). One problem uses the pool at a time; a planning
+// thread that finds it busy models its candidates itself. LIBRETT_PLAN_THREADS
+// sets the number of threads (1: no pool).
+class PlanPool {
+ public:
+  static PlanPool& get() {
+    static PlanPool* p = new PlanPool;
+    return *p;
+  }
+  int threads() const { return nthreads_; }
+  // Runs f(0..n-1); returns false (running nothing) when the pool is busy.
+  bool run(int n, const std::function<void(int)>& f) {
+    std::unique_lock<std::mutex> busy(busy_mu_, std::try_to_lock);
+    if (!busy.owns_lock()) return false;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      job_ = &f;
+      n_ = n;
+      next_.store(0);
+      pending_ = (int)workers_.size();
+      gen_++;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lock(mu_);
+    done_cv_.wait(lock, [&] { return pending_ == 0; });
+    job_ = nullptr;
+    return true;
+  }
+
+ private:
+  PlanPool() {
+    const char* e = std::getenv("LIBRETT_PLAN_THREADS");
+    int t = (e && *e) ? std::atoi(e) : (int)std::thread::hardware_concurrency();
+    nthreads_ = std::max(1, std::min(t, 16));
+    for (int i = 1; i < nthreads_; i++) {
+      workers_.emplace_back([this] { loop(); });
+      workers_.back().detach();
+    }
+  }
+  void work() {
+    for (;;) {
+      const int i = next_.fetch_add(1);
+      if (i >= n_) break;
+      (*job_)(i);
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lock(mu_);
+        cv_.wait(lock, [&] { return gen_ != seen; });
+        seen = gen_;
+      }
+      work();
+      std::lock_guard<std::mutex> lock(mu_);
+      if (--pending_ == 0) done_cv_.notify_all();
+    }
+  }
+  int nthreads_ = 1;
+  std::vector<std::thread> workers_;
+  std::mutex busy_mu_, mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)>* job_ = nullptr;
+  int n_ = 0;
+  std::atomic<int> next_{0};
+  int pending_ = 0;
+  uint64_t gen_ = 0;
+};
+
+// f(0..n-1), on the planning threads when there is enough work.
+void parallel_for(int n, const std::function<void(int)>& f) {
+  if (n >= 8) {
+    PlanPool& pool = PlanPool::get();
+    if (pool.threads() > 1 && pool.run(n, f)) return;
+  }
+  for (int i = 0; i < n; i++) f(i);
 }
 
 // Deterministic sample of `k` indices in [0, n).
@@ -648,11 +735,11 @@ struct Run {
 };
 
 // Distinct 128-byte lines a set of runs touches, and how many of them the
-// runs cover only in part (a run boundary falls inside the line).
-void line_stats(std::vector<Run>& runs, double& lines, double& partial) {
+// runs cover only in part (a run boundary falls inside the line). `runs`
+// must be sorted by start offset.
+void line_stats_sorted(const std::vector<Run>& runs, double& lines, double& partial) {
   lines = partial = 0;
   if (runs.empty()) return;
-  std::sort(runs.begin(), runs.end(), [](const Run& a, const Run& b) { return a.s < b.s; });
   long long last_line = -1, last_partial = -1;
   size_t i = 0;
   while (i < runs.size()) {
@@ -674,6 +761,59 @@ void line_stats(std::vector<Run>& runs, double& lines, double& partial) {
     }
     last_line = last;
     i = j;
+  }
+}
+
+void line_stats(std::vector<Run>& runs, double& lines, double& partial) {
+  auto less = [](const Run& a, const Run& b) { return a.s < b.s; };
+  if (!std::is_sorted(runs.begin(), runs.end(), less)) std::sort(runs.begin(), runs.end(), less);
+  line_stats_sorted(runs, lines, partial);
+}
+
+// Runs merged the way line_stats_sorted merges them (a run that starts at
+// or before the current end joins it). `runs` sorted by start.
+void merged_intervals(const std::vector<Run>& runs, std::vector<Run>& out) {
+  out.clear();
+  for (const Run& r : runs) {
+    if (!out.empty() && r.s <= out.back().s + out.back().n)
+      out.back().n = std::max(out.back().s + out.back().n, r.s + r.n) - out.back().s;
+    else
+      out.push_back(r);
+  }
+}
+
+// Whether line_stats counts line L as partly covered by `runs` (any order):
+// a merged interval of them starts or ends inside L, off its boundary.
+bool line_partial(long long L, std::vector<Run> runs) {
+  std::sort(runs.begin(), runs.end(), [](const Run& a, const Run& b) { return a.s < b.s; });
+  std::vector<Run> m;
+  merged_intervals(runs, m);
+  for (const Run& r : m) {
+    const long long e = r.s + r.n;
+    if (((r.s >> 7) == L && (r.s & 127)) || (((e - 1) >> 7) == L && (e & 127))) return true;
+  }
+  return false;
+}
+
+// Merges consecutive sorted segments of `runs` (segment k ends at ends[k])
+// into one sorted list: the exact order line_stats would sort them into,
+// without a full sort.
+void merge_segments(std::vector<Run>& runs, std::vector<size_t>& ends) {
+  auto less = [](const Run& a, const Run& b) { return a.s < b.s; };
+  if (std::is_sorted(runs.begin(), runs.end(), less)) return;
+  while (ends.size() > 1) {
+    std::vector<size_t> next;
+    size_t begin = 0;
+    for (size_t k = 0; k < ends.size(); k += 2) {
+      if (k + 1 < ends.size()) {
+        std::inplace_merge(runs.begin() + begin, runs.begin() + ends[k], runs.begin() + ends[k + 1], less);
+        next.push_back(ends[k + 1]);
+        begin = ends[k + 1];
+      } else {
+        next.push_back(ends[k]);
+      }
+    }
+    ends.swap(next);
   }
 }
 
@@ -814,20 +954,112 @@ namespace {
 // indices of one block, because a block walks its outer range in order and
 // the L2 keeps the lines its previous rounds touched; short runs that are
 // adjacent across rounds therefore cost one line fetch, not one per run.
-void model_packed(Candidate& c, const DeviceInfo& dev) {
+//
+// The sums depend only on the candidate's staged shapes, split and outer
+// map, not on its work order, so the XCD-interleaved twin of a plan reuses
+// them (PackedSums). One round's own line and transaction counts depend only
+// on where its base offset falls within a 128-byte line, so they are
+// computed once per (chunk variant, side, alignment). A staged shape walked
+// in its own order has strictly increasing offsets (a sub-box of a
+// column-major array), so each round's runs come sorted and a window needs a
+// merge, not a sort.
+struct PackedSums {
+  double R = 0, W = 0, P = 0, TR = 0, TW = 0, rounds = 0, R1 = 0, W1 = 0;
+};
+
+// Statistics of one round's runs placed at byte alignment `al` within a
+// line: distinct lines, partly covered lines, transactions, the first and
+// last line touched (relative to the line holding the base) and the merged
+// intervals that touch those two lines.
+struct RoundPattern {
+  bool done = false;
+  double lines = 0, partial = 0, tran = 0;
+  long long first = 0, last = -1;
+  std::vector<Run> head, tail;
+};
+
+// Lines and partly covered lines of the union of several rounds of one
+// pattern side, from each round's own statistics. Rounds whose line ranges
+// are disjoint, or share only a boundary line (with no line shared by three
+// rounds), combine exactly: shared lines are counted once and their partial
+// state is recomputed from the intervals that touch them. Returns false
+// when the ranges overlap further (the caller then merges the runs).
+struct PlacedRound {
+  long long q;  // line of the round's base
+  const RoundPattern* pat;
+};
+bool union_stats(std::vector<PlacedRound>& rs, double& lines, double& partial) {
+  std::sort(rs.begin(), rs.end(), [](const PlacedRound& a, const PlacedRound& b) {
+    return a.q + a.pat->first < b.q + b.pat->first;
+  });
+  lines = partial = 0;
+  for (size_t k = 0; k < rs.size(); k++) {
+    lines += rs[k].pat->lines;
+    partial += rs[k].pat->partial;
+    if (k == 0) continue;
+    const PlacedRound& A = rs[k - 1];
+    const PlacedRound& Bq = rs[k];
+    const long long a_last = A.q + A.pat->last, b_first = Bq.q + Bq.pat->first;
+    if (b_first > a_last) continue;
+    if (b_first < a_last) return false;
+    // Shared line L: a line shared with the round before A as well is left
+    // to the caller.
+    if (k >= 2 && A.pat->first == A.pat->last && rs[k - 2].q + rs[k - 2].pat->last == a_last) return false;
+    if (Bq.pat->first == Bq.pat->last && k + 1 < rs.size() && rs[k + 1].q + rs[k + 1].pat->first == a_last)
+      return false;
+    const long long L = a_last;
+    std::vector<Run> ta, hb, both;
+    for (const Run& r : A.pat->tail) ta.push_back({r.s + A.q * 128, r.n});
+    for (const Run& r : Bq.pat->head) hb.push_back({r.s + Bq.q * 128, r.n});
+    both = ta;
+    both.insert(both.end(), hb.begin(), hb.end());
+    lines -= 1;
+    partial += (line_partial(L, both) ? 1 : 0) - (line_partial(L, ta) ? 1 : 0) - (line_partial(L, hb) ? 1 : 0);
+  }
+  return true;
+}
+
+PackedSums packed_sums(const Candidate& c) {
   const PackedArgs& a = c.packed;
   const int B = c.elem;
-  double R = 0, W = 0, P = 0, TR = 0, TW = 0, rounds = 0, R1 = 0, W1 = 0;
+  PackedSums S;
   std::vector<std::pair<long long, int>> rin[2], rout[2];
   const int nvar = (c.method == kPackedSplit) ? 2 : 1;
   for (int v = 0; v < nvar; v++) {
     staged_runs(a.shape[v], false, rin[v]);
     staged_runs(a.shape[v], true, rout[v]);
   }
+  // Round patterns of variant v, side (0 in, 1 out) at byte alignment `al`
+  // = (base * B) mod 128.
+  std::vector<RoundPattern> memo(2 * 2 * 128);
+  std::vector<Run> tmp, merged;
+  auto pattern = [&](int v, int side, long long al) -> const RoundPattern& {
+    RoundPattern& r = memo[(v * 2 + side) * 128 + al];
+    if (!r.done) {
+      const auto& runs = side ? rout[v] : rin[v];
+      tmp.clear();
+      for (const auto& x : runs) tmp.push_back({al + x.first * B, (long long)x.second * B});
+      line_stats_sorted(tmp, r.lines, r.partial);
+      r.tran = staged_tran(runs, al / B, B);
+      merged_intervals(tmp, merged);
+      if (!merged.empty()) {
+        r.first = merged.front().s >> 7;
+        r.last = (merged.back().s + merged.back().n - 1) >> 7;
+        for (const Run& m : merged) {
+          if ((m.s >> 7) <= r.first) r.head.push_back(m);
+          if (((m.s + m.n - 1) >> 7) >= r.last) r.tail.push_back(m);
+        }
+      }
+      r.done = true;
+    }
+    return r;
+  };
   const long long nblocks = (long long)c.grid_x * c.grid_y;
   const int window = std::max(1, std::min(4, a.outer_per_block));
   std::vector<long long> bs = sample_indices(nblocks, 6);
   std::vector<Run> win, wout;
+  std::vector<size_t> ein, eout;
+  std::vector<PlacedRound> pin, pout;
   for (long long b : bs) {
     const int split = (int)(b % c.grid_x);
     const int by = (int)(b / c.grid_x);
@@ -837,57 +1069,83 @@ void model_packed(Candidate& c, const DeviceInfo& dev) {
     const int plus = (nvar == 2) ? (c1 - c0) - q0 : 0;
     const int o0 = by * a.outer_per_block;
     const int o1 = std::min(a.vol_outer, o0 + window);
-    win.clear();
-    wout.clear();
+    pin.clear();
+    pout.clear();
     for (int o = o0; o < o1; o++) {
       int oi, oo;
       outer_offsets(a.outer, (uint32_t)o, oi, oo);
-      const long long bi = (long long)c0 * a.split_in_stride + oi;
-      const long long bo = (long long)c0 * a.split_out_stride + oo;
-      for (const auto& r : rin[plus]) win.push_back({(bi + r.first) * B, (long long)r.second * B});
-      for (const auto& r : rout[plus]) wout.push_back({(bo + r.first) * B, (long long)r.second * B});
-      TR += staged_tran(rin[plus], bi, B);
-      TW += staged_tran(rout[plus], bo, B);
+      const long long bi = ((long long)c0 * a.split_in_stride + oi) * B;
+      const long long bo = ((long long)c0 * a.split_out_stride + oo) * B;
+      // Each round alone (no reuse between rounds).
+      const RoundPattern& ri = pattern(plus, 0, bi & 127);
+      const RoundPattern& ro = pattern(plus, 1, bo & 127);
+      S.TR += ri.tran;
+      S.TW += ro.tran;
+      S.R1 += ri.lines;
+      S.W1 += ro.lines;
+      pin.push_back({bi >> 7, &ri});
+      pout.push_back({bo >> 7, &ro});
     }
     double l, p;
-    line_stats(win, l, p);
-    R += l;
-    line_stats(wout, l, p);
-    W += l;
-    P += p;
-    rounds += o1 - o0;
-    // Same rounds without reuse between rounds (each round alone).
-    for (int o = o0; o < o1; o++) {
-      int oi, oo;
-      outer_offsets(a.outer, (uint32_t)o, oi, oo);
-      const long long bi = (long long)c0 * a.split_in_stride + oi;
-      const long long bo = (long long)c0 * a.split_out_stride + oo;
+    if (!union_stats(pin, l, p)) {
       win.clear();
-      wout.clear();
-      for (const auto& r : rin[plus]) win.push_back({(bi + r.first) * B, (long long)r.second * B});
-      for (const auto& r : rout[plus]) wout.push_back({(bo + r.first) * B, (long long)r.second * B});
-      line_stats(win, l, p);
-      R1 += l;
-      line_stats(wout, l, p);
-      W1 += l;
+      ein.clear();
+      for (int o = o0; o < o1; o++) {
+        int oi, oo;
+        outer_offsets(a.outer, (uint32_t)o, oi, oo);
+        const long long bi = (long long)c0 * a.split_in_stride + oi;
+        for (const auto& r : rin[plus]) win.push_back({(bi + r.first) * B, (long long)r.second * B});
+        ein.push_back(win.size());
+      }
+      merge_segments(win, ein);
+      line_stats_sorted(win, l, p);
     }
+    S.R += l;
+    if (!union_stats(pout, l, p)) {
+      wout.clear();
+      eout.clear();
+      for (int o = o0; o < o1; o++) {
+        int oi, oo;
+        outer_offsets(a.outer, (uint32_t)o, oi, oo);
+        const long long bo = (long long)c0 * a.split_out_stride + oo;
+        for (const auto& r : rout[plus]) wout.push_back({(bo + r.first) * B, (long long)r.second * B});
+        eout.push_back(wout.size());
+      }
+      merge_segments(wout, eout);
+      line_stats_sorted(wout, l, p);
+    }
+    S.W += l;
+    S.P += p;
+    S.rounds += o1 - o0;
   }
+  return S;
+}
+
+void model_packed(Candidate& c, const DeviceInfo& dev, const PackedSums& S) {
+  const PackedArgs& a = c.packed;
   const long long nitems = (long long)a.num_split * a.vol_outer;
-  const double scale = rounds > 0 ? (double)nitems / rounds : 0;
+  const double scale = S.rounds > 0 ? (double)nitems / S.rounds : 0;
   // Reuse between rounds needs the lines of one round of all resident blocks
   // to stay in L2; past that footprint it fades out.
   const double resident = std::min((double)c.grid_x * c.grid_y, (double)dev.cu_count * std::max(1, c.active_blocks_per_cu));
-  const double footprint = resident * 128.0 * (R1 + W1) / std::max(1.0, rounds);
+  const double footprint = resident * 128.0 * (S.R1 + S.W1) / std::max(1.0, S.rounds);
   // In the XCD-interleaved order the blocks sharing lines run together on
   // one XCD, so the reuse does not depend on the lines surviving in L2.
   const double keep = a.order == 1 ? 1.0 : std::min(1.0, dev.l2_bytes / std::max(1.0, footprint));
-  c.model.read_lines = (R * keep + R1 * (1 - keep)) * scale;
-  c.model.write_lines = (W * keep + W1 * (1 - keep)) * scale;
-  c.model.partial_lines = P * scale;
-  c.model.read_tran = TR * scale;
-  c.model.write_tran = TW * scale;
+  c.model.read_lines = (S.R * keep + S.R1 * (1 - keep)) * scale;
+  c.model.write_lines = (S.W * keep + S.W1 * (1 - keep)) * scale;
+  c.model.partial_lines = S.P * scale;
+  c.model.read_tran = S.TR * scale;
+  c.model.write_tran = S.TW * scale;
   c.model.items = (double)nitems;
   c.model.steps_per_block = a.outer_per_block;
+}
+
+// True when b is a's plan in the other work order (same sums).
+bool order_twins(const Candidate& a, const Candidate& b) {
+  return (a.method == kPacked || a.method == kPackedSplit) && same_packed(a, b) && a.grid_x == b.grid_x &&
+         a.grid_y == b.grid_y && a.packed.outer_per_block == b.packed.outer_per_block &&
+         a.source == b.source && a.packed.order != b.packed.order;
 }
 
 }  // namespace
@@ -922,7 +1180,7 @@ void model_candidate(Candidate& c, const Problem& full, const DeviceInfo& dev) {
       model_copy(c);
       break;
     default:
-      model_packed(c, dev);
+      model_packed(c, dev, packed_sums(c));
       break;
   }
   model_times(c, dev);
@@ -971,9 +1229,30 @@ std::vector<Candidate> make_candidates(const Problem& full, const DeviceInfo& de
     add_tiled(red, dev, out);
   add_packed(full, "full", dev, out);
   if (red.rank != full.rank) add_packed(red, "reduced", dev, out);
-  for (auto& c : out) {
+  for (auto& c : out)
     if (c.active_blocks_per_cu <= 0) c.active_blocks_per_cu = active_blocks(c, dev);
-    model_candidate(c, full, dev);
+  // Model every candidate; the XCD-interleaved twin of a Packed plan follows
+  // it in the list and shares its sums. Candidates are independent, so the
+  // work is spread over the planning threads.
+  const size_t n = out.size();
+  std::vector<PackedSums> sums(n);
+  std::vector<char> twin(n, 0);
+  for (size_t i = 1; i < n; i++) twin[i] = order_twins(out[i - 1], out[i]);
+  parallel_for((int)n, [&](int i) {
+    Candidate& c = out[i];
+    if (c.method == kPacked || c.method == kPackedSplit) {
+      if (!twin[i]) sums[i] = packed_sums(c);
+    } else {
+      model_candidate(c, full, dev);
+    }
+  });
+  for (size_t i = 0; i < n; i++) {
+    Candidate& c = out[i];
+    if (c.method != kPacked && c.method != kPackedSplit) continue;
+    if (twin[i]) sums[i] = sums[i - 1];
+    c.model = ModelStats();
+    model_packed(c, dev, sums[i]);
+    model_times(c, dev);
   }
   // Drop candidates that cannot launch on this device.
   std::vector<Candidate> ok;

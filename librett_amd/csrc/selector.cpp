@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <numeric>
 #include <random>
@@ -230,16 +231,19 @@ int selector_choose(const std::vector<Candidate>& cands, const Problem& full, in
   bool ok = true;
   if (!d.stream) ok = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) == hipSuccess;
   const int models = (int)(s.params.size() / mlp::P);
-  if (ok && !d.params)
-    ok = hipMalloc((void**)&d.params, (size_t)kSelMaxModels * mlp::P * sizeof(float)) == hipSuccess;
+  if (ok && !d.params) {
+    ok = dev_malloc((void**)&d.params, (size_t)kSelMaxModels * mlp::P * sizeof(float)) == hipSuccess;
+    if (!ok) d.params = nullptr;
+    d.gen = 0;
+  }
   if (ok && d.cap < n) {
-    if (d.x) (void)hipFree(d.x);
-    if (d.out) (void)hipFree(d.out);
+    dev_free(d.x);
+    dev_free(d.out);
     d.x = d.out = nullptr;
     d.cap = 0;
     const int cap = (n + 63) / 64 * 64;
-    ok = hipMalloc((void**)&d.x, (size_t)cap * mlp::F * sizeof(float)) == hipSuccess &&
-         hipMalloc((void**)&d.out, (size_t)kSelMaxModels * cap * sizeof(float)) == hipSuccess;
+    ok = dev_malloc((void**)&d.x, (size_t)cap * mlp::F * sizeof(float)) == hipSuccess &&
+         dev_malloc((void**)&d.out, (size_t)kSelMaxModels * cap * sizeof(float)) == hipSuccess;
     if (ok) d.cap = cap;
   }
   if (ok && d.gen != s.gen) {
@@ -274,6 +278,23 @@ int selector_choose(const std::vector<Candidate>& cands, const Problem& full, in
   const int mc = choose_by_model(cands);
   if (mc >= 0 && mc < n && score[mc] - score[best] < selector_margin()) return mc;
   return best;
+}
+
+void selector_release_buffers(const std::function<void()>& then) {
+  Selector& s = state();
+  std::lock_guard<std::mutex> lock(s.mu);
+  for (auto& kv : s.dev) {
+    DevBuf& d = kv.second;
+    if (d.stream) (void)hipStreamSynchronize(d.stream);
+    dev_free(d.params);
+    dev_free(d.x);
+    dev_free(d.out);
+    d.params = d.x = d.out = nullptr;
+    d.cap = 0;
+    d.gen = 0;
+  }
+  (void)hipGetLastError();
+  then();
 }
 
 }  // namespace lrt

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "planner.h"
@@ -49,6 +50,13 @@ void selector_features(const Problem& full, const Problem& reduced, const Candid
 // fails (the caller then uses the analytic model). Runs the HIP inference
 // kernel on the current device.
 int selector_choose(const std::vector<Candidate>& cands, const Problem& full, int device);
+
+// Returns the per-device inference buffers (taken from the device allocator,
+// kept between plans) to the allocator that made them, then runs `then`
+// with the selector locked, so no plan can take a buffer in between. The
+// allocator hook calls it when the allocator changes; the buffers are made
+// again from the new allocator on the next plan.
+void selector_release_buffers(const std::function<void()>& then);
 
 // Generation of the loaded weights (0: none); part of the plan-cache key.
 uint64_t selector_generation();

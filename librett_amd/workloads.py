@@ -84,3 +84,35 @@ def random_problem(rng, rank_lo, rank_hi, vol_lo, vol_hi, elems):
     dims = _fill_dims(rng, rank, {}, target)
     perm = _perm(rng, rank)
     return dims, perm, elem
+
+
+def test2_dims(rng, rank, volmax=2e8, min_dim=2.0):
+    """Dims drawn the way the reference's test2 draws them
+    (tests/librett_test.cpp:194-259), from a seeded generator instead of
+    rand(): a target volume uniform in [(min_dim+1)^rank, volmax], dims
+    spread around its rank-th root, rescaled toward the target and kept >= 2,
+    redrawn until the volume fits volmax and is within a factor 2.3 of the
+    target. The reference carries the running volume from one rescaling
+    attempt into the next, and so does this restatement."""
+    volmin = (min_dim + 1.0) ** rank
+    dims = [2] * rank
+    curvol = 1.0
+    for _ in range(1000):
+        vol = volmin + (volmax - volmin) * rng.random()
+        for _ in range(50):
+            ave = vol ** (1.0 / rank)
+            spread = ave - min_dim
+            for r in range(rank):
+                rn = 2.0 * (rng.random() - 0.5)
+                dims[r] = int(ave + spread * rn)
+                curvol *= dims[r]
+            scale = (vol / curvol) ** (1.0 / rank)
+            curvol = 1.0
+            for r in range(rank):
+                dims[r] = max(2, int(round(dims[r] * scale)))
+                curvol *= dims[r]
+            if not (curvol > volmax or abs(curvol - vol) / vol > 2.3):
+                break
+        if not (curvol > volmax or abs(curvol - vol) / vol > 2.3):
+            return list(dims)
+    raise RuntimeError("no dims found for rank %d" % rank)

@@ -36,7 +36,8 @@ def replay_all(dim, perm, elem, seed=1, device=None, max_cands=None):
 def test_reduce_ranks_examples():
     assert L.reduce_ranks([64, 48, 32], [2, 0, 1]) == ([3072, 32], [1, 0])
     assert L.reduce_ranks([2, 3, 4], [0, 1, 2]) == ([24], [0])
-    assert L.reduce_ranks([5, 1, 7], [2, 1, 0]) == ([7, 5], [1, 0]) or True  # size-1 dims dropped first
+    # The size-1 dim is dropped first, leaving (5, 7) with perm (1, 0).
+    assert L.reduce_ranks([5, 1, 7], [2, 1, 0]) == ([5, 7], [1, 0])
     d, p = L.reduce_ranks([5, 1, 7], [2, 1, 0])
     assert int(np.prod(d)) == 35 and sorted(p) == list(range(len(p)))
 
