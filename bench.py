@@ -15,6 +15,14 @@ metric is 2*bytes/t summed over the step (the reference's librettTimer count,
 src/Timer.cpp:151). The line also carries a same-run hipMemcpyAsync D2D
 ceiling, the roofline of the dominant kernel (HIP events on the plan's
 stream) and a CPU baseline (the OpenMP host permutation on a bounded sample).
+
+Unless --quick is given, the same line then carries the other workloads,
+each with its outputs checked on the device: "batched" (4096 uneven
+problems in one persistent graph launch vs a per-call librettExecute loop),
+"serving" (Poisson arrivals through the continuous-batching engine: warm and
+cold plan caches, p50/p99 over a load sweep, per-call baselines),
+"large_random" (48 seeded ~256 MiB cases, librettPlan vs librettPlanMeasure)
+and "selector" (held-out slowdown of the plan choice vs exhaustive timing).
 """
 import argparse
 import json
@@ -64,6 +72,11 @@ def main():
                          "through the continuous-batching engine; selector: learned plan selector vs analytic "
                          "model on held-out problems")
     ap.add_argument("--heldout", type=int, default=2000, help="selector workload: held-out problems")
+    ap.add_argument("--selector-problems", type=int, default=200,
+                    help="held-out problems of the selector part of the default run")
+    ap.add_argument("--quick", action="store_true",
+                    help="default run: only the seven named cases (no batched / serving / large-random / selector "
+                         "parts)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "RUN_LOG"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
                          "'bench.py --steps 1 --warmup 0 --no-cpu' (RUN_LOG: that run's stderr, for the plans) "
@@ -92,14 +105,35 @@ def main():
     stream = hip.stream_create()
     if args.analytic:
         L.selector_set(None, None, None)
-    if args.workload == "batched":
-        return bench_batched(args, stream)
-    if args.workload == "serving":
-        return bench_serving(args, stream)
-    if args.workload == "selector":
-        return bench_selector(args, stream)
-    if args.workload == "large-random":
-        return bench_large_random(args, stream)
+    sub = {"batched": bench_batched, "serving": bench_serving, "selector": bench_selector,
+           "large-random": bench_large_random}
+    if args.workload in sub:
+        print(json.dumps(sub[args.workload](args, stream)), flush=True)
+        return
+    line = bench_named(args, stream)
+    if not args.quick:
+        # The other workloads, each checked on the device, in the same line.
+        t0 = time.perf_counter()
+        line["batched"] = summarize_batched(bench_batched(args, stream))
+        log("batched workload done (%.0f s)" % (time.perf_counter() - t0))
+        t0 = time.perf_counter()
+        line["serving"] = summarize_serving(bench_serving(args, stream))
+        log("serving workload done (%.0f s)" % (time.perf_counter() - t0))
+        t0 = time.perf_counter()
+        line["large_random"] = summarize_large_random(bench_large_random(args, stream))
+        log("large-random workload done (%.0f s)" % (time.perf_counter() - t0))
+        t0 = time.perf_counter()
+        args.heldout = args.selector_problems
+        line["selector"] = summarize_selector(bench_selector(args, stream))
+        log("selector workload done (%.0f s)" % (time.perf_counter() - t0))
+    hip.stream_destroy(stream)
+    print(json.dumps(line), flush=True)
+
+
+def bench_named(args, stream):
+    """The seven named ~256 MiB cases: the headline line."""
+    import librett_amd as L
+    from librett_amd import data, hip
     selection = ("librettPlanMeasure" if args.measure else
                  "learned selector (librettPlan)" if L.selector_generation() else "gfx950 analytic model (librettPlan)")
     if args.exact_counts and not args.measure:
@@ -230,7 +264,6 @@ def main():
         L.librettDestroy(c["h"])
         c["a"].free()
         c["b"].free()
-    hip.stream_destroy(stream)
 
     line = {
         "metric": "transpose GB/s (2*bytes/t) vs HBM peak",
@@ -268,9 +301,12 @@ def main():
             "avg_launch_ms": round(dom_ms / dom_launches, 4),
         },
         "cpu_baseline": cpu,
+        "reference_hip_build": "not measured: the reference sources are not on the GPU box (no LIBRETT_REF_DIR)",
+        "plan_ms": {"%s %s %dB" % (tuple(c["dim"]), tuple(c["perm"]), c["elem"]): round(c["plan_s"] * 1e3, 2)
+                    for c in cases},
         "cases": per_case,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_large_random(args, stream):
@@ -351,7 +387,17 @@ def bench_large_random(args, stream):
     }
     A.free()
     B.free()
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def summarize_large_random(r):
+    worst = sorted(r["cases"], key=lambda c: -c["plan_over_measure"])[:5]
+    return {"gbs": r["value"], "measure_gbs": r["measure_gbs"], "d2d_gbs": r["d2d_gbs"],
+            "frac_of_d2d": r["frac_of_d2d"], "plan_over_measure_time_median": r["plan_over_measure_time_median"],
+            "plan_over_measure_time_max": r["plan_over_measure_time_max"], "by_class": r["by_class"],
+            "checked": "every output checked on the device (index pattern)", "cases": len(r["cases"]),
+            "worst": [{k: c[k] for k in ("cls", "elem", "seed", "dim", "perm", "plan_gbs", "measure_gbs",
+                                         "plan_over_measure")} for c in worst]}
 
 
 def bench_selector(args, stream):
@@ -392,22 +438,31 @@ def bench_selector(args, stream):
         "inference_ms_all_candidates": round(infer * 1e3, 2),
         "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
+    A.free()
+    B.free()
+    return line
+
+
+def summarize_selector(r):
+    return {"problems": r["config"]["workload"], "candidates_timed": r["config"]["candidates_timed"],
+            "librettPlan_slowdown": r["selector"], "selector_alone_slowdown": r["selector_alone"],
+            "analytic_slowdown": r["analytic"], "margin": r["selector_margin"]}
 
 
 def batched_problems(n, seed=20261202):
     """Uneven batch: rank 3-5, volume log-uniform 1e3..1e6 elements, float64
     or complex128, random non-identity permutation per problem."""
     from librett_amd import workloads as W
-    rng = np.random.default_rng(seed)
-    return [W.random_problem(rng, 3, 5, 1e3, 1e6, [8, 16]) for _ in range(n)]
+    return W.batched_problems(n, seed)
 
 
 def bench_batched(args, stream):
     """Batched workload: the whole batch in one persistent graph launch vs a
-    loop of per-problem librettExecute calls on the same buffers."""
+    loop of per-problem librettExecute calls on the same buffers. Inputs hold
+    the index pattern, and every output of the timed batch is checked on the
+    device afterwards."""
     import librett_amd as L
-    from librett_amd import data, hip
+    from librett_amd import hip
     probs = batched_problems(args.problems)
     sizes = [int(np.prod(d)) * e for d, p, e in probs]
     offs, o = [], 0
@@ -419,14 +474,8 @@ def bench_batched(args, stream):
     log("batched: %d problems, %.2f GB algorithmic, %.2f GB per side" % (len(probs), alg / 1e9, total / 1e9))
     A = hip.DeviceBuffer(total)
     B = hip.DeviceBuffer(total)
-    chunk = 256 << 20
-    src = data.uniform(chunk // 8, 8, 7).view(np.uint8)
-    hip.htod(A.ptr, src[:min(chunk, total)])
-    done = min(chunk, total)
-    while done < total:
-        n = min(done, total - done)
-        hip.memcpy_d2d_async(A.ptr + done, A.ptr, n, stream)
-        done += n
+    for off, s_ in zip(offs, sizes):
+        L.check_fill(A.ptr + off, s_, stream)
     hip.memset(B.ptr, 0, total)
     hip.stream_sync(stream)
     ins = [A.ptr + o for o in offs]
@@ -445,6 +494,14 @@ def bench_batched(args, stream):
         e1.synchronize()
         return (time.perf_counter() - t0) / steps, e1.elapsed_ms(e0) * 1e-3 / steps
 
+    def check_all(what):
+        hip.stream_sync(stream)
+        bad = [i for i, ((d, p, e), off) in enumerate(zip(probs, offs))
+               if L.check_transpose(B.ptr + off, d, p, e, stream)[0]]
+        if bad:
+            raise SystemExit("batched (%s): %d wrong outputs, first %s" % (what, len(bad), probs[bad[0]]))
+        hip.memset(B.ptr, 0, total)
+
     # Per-call loop of librettExecute with plans built beforehand.
     t0 = time.perf_counter()
     handles = []
@@ -457,8 +514,10 @@ def bench_batched(args, stream):
 
     def loop():
         for h, i, o_ in zip(handles, ins, outs):
-            L.librettExecute(h, i, o_)
+            if L.librettExecute(h, i, o_) != L.LIBRETT_SUCCESS:
+                raise SystemExit("librettExecute failed in the per-call loop")
     wall_loop, ev_loop = timed(loop, max(1, args.steps // 4), 1)
+    check_all("per-call loop")
     log("per-call loop: %.2f ms/step (%.1f GB/s), planning %.1f s" % (wall_loop * 1e3, alg / wall_loop / 1e9,
                                                                        plan_loop_s))
     res = {}
@@ -468,14 +527,22 @@ def bench_batched(args, stream):
         bt.set_buffers(ins, outs)
         info = bt.info()
         build_s = time.perf_counter() - t0
-        wall_b, ev_b = timed(lambda: bt.execute(use_graph=True), args.steps, args.warmup)
-        wall_n, ev_n = timed(lambda: bt.execute(use_graph=False), args.steps, args.warmup)
+
+        def run(graph):
+            if bt.execute(use_graph=graph) != L.LIBRETT_SUCCESS:
+                raise SystemExit("batch execute failed")
+        wall_b, ev_b = timed(lambda: run(True), args.steps, args.warmup)
+        check_all("graph, tune %d" % tune)
+        wall_n, ev_n = timed(lambda: run(False), args.steps, args.warmup)
+        check_all("plain launch, tune %d" % tune)
         res[tune] = dict(wall=wall_b, ev=ev_b, wall_plain=wall_n, info=info, build_s=build_s)
         log("batch tune=%d: graph %.3f ms (%.1f GB/s), plain launch %.3f ms; units %d grid %d; methods %s; "
             "plan %.2f s tune %.2f s" % (tune, wall_b * 1e3, alg / wall_b / 1e9, wall_n * 1e3, info["total_units"],
                                         info["grid"], info["method_count"], info["plan_seconds"],
                                         info["tune_seconds"]))
         bt.destroy()
+    # Same-buffer copy ceiling: every problem's bytes copied in one pass.
+    d2d = timed(lambda: hip.memcpy_d2d_async(B.ptr, A.ptr, total, stream), args.steps, 1)[0]
     best_tune = min(res, key=lambda k: res[k]["wall"])
     wall = res[best_tune]["wall"]
     value = alg / wall / 1e9
@@ -493,40 +560,91 @@ def bench_batched(args, stream):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64+c128 (opaque 8/16-byte moves)",
-        "data": "synthetic",
+        "data": "synthetic (index pattern; every output checked on the device)",
         "config": {"workload": "batched: %d uneven transposes, rank 3-5, vol 1e3-1e6, f64/c128" % len(probs),
                    "bytes_per_step": alg, "autotune_candidates": best_tune},
         "per_call_loop_gbs": round(alg / wall_loop / 1e9, 1),
         "speedup_vs_per_call": round(wall_loop / wall, 2),
+        "d2d_copy_gbs": round(2 * total / d2d / 1e9, 1),
+        "frac_of_d2d": round(value / (2 * total / d2d / 1e9), 3),
         "model_only_gbs": round(alg / res[0]["wall"] / 1e9, 1),
         "autotuned_gbs": round(alg / res[args.tune]["wall"] / 1e9, 1),
         "plain_launch_gbs": round(alg / res[best_tune]["wall_plain"] / 1e9, 1),
+        "autotune_seconds": round(res[args.tune]["info"]["tune_seconds"], 2),
         "batch_info": res[best_tune]["info"],
     }
     A.free()
     B.free()
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def summarize_batched(r):
+    keys = ("value", "ms_per_step", "per_call_loop_gbs", "speedup_vs_per_call", "d2d_copy_gbs", "frac_of_d2d",
+            "model_only_gbs", "autotuned_gbs", "plain_launch_gbs", "autotune_seconds")
+    out = {("gbs" if k == "value" else k): r[k] for k in keys}
+    out["workload"] = r["config"]["workload"]
+    out["checked"] = r["data"]
+    return out
+
+
+def serving_checked_pass(srv, stream, n, seed):
+    """Requests of the serving distribution submitted from the host with
+    index-pattern inputs in the arena, all in flight together; every output
+    checked on the device. Returns the number of requests checked."""
+    import librett_amd as L
+    from librett_amd import hip, workloads as W
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n):
+        e = int(rng.choice([4, 8, 16]))
+        nb = float(np.exp(rng.uniform(np.log(4096), np.log(64 << 20))))
+        r = int(rng.integers(2, 7))
+        reqs.append(W.random_problem(rng, r, r, max(2.0, nb / e), max(2.0, nb / e), [e]))
+    for wave in range(0, n, 100):
+        part = reqs[wave:wave + 100]
+        bufs = []
+        for d, p, e in part:
+            nb = int(np.prod(d)) * e
+            pin, pout = srv.alloc(nb), srv.alloc(nb)
+            if pin is None or pout is None:
+                raise SystemExit("serving check: arena full")
+            L.check_fill(pin, nb, stream)
+            bufs.append((pin, pout))
+        hip.stream_sync(stream)
+        tickets = [srv.submit(d, p, e, pi, po) for (d, p, e), (pi, po) in zip(part, bufs)]
+        for t in tickets:
+            srv.wait(t)
+        for (d, p, e), (pi, po) in zip(part, bufs):
+            bad = L.check_transpose(po, d, p, e, stream)[0]
+            if bad:
+                raise SystemExit("serving: wrong output for %s %s %d B (%d elements)" % (d, p, e, bad))
+            srv.free(pi)
+            srv.free(po)
+    return n
 
 
 def bench_serving(args, stream):
     """Serving workload: Poisson arrivals of requests 4 KiB-64 MiB (log-uniform),
     rank 2-6, f32/f64/c128, random permutations, through the continuous-
-    batching engine with a device arena budget; compared with a per-call loop
-    of librettExecute over the same request mix."""
+    batching engine; admission reserves input and output bytes in the device
+    arena. Measured with a warm plan cache (shapes from a catalog planned up
+    front) and with a new shape per request (planned on arrival by the client
+    threads); compared with per-call librettExecute loops over the same mix."""
     import librett_amd as L
     from librett_amd import hip
     pool_bytes = 1 << 30
     pool = hip.DeviceBuffer(pool_bytes)
     hip.memset(pool.ptr, 5, pool_bytes)
     srv = L.Server(int(args.arena_gb * (1 << 30)), slots=args.slots, max_batch=args.max_batch)
-    kw = dict(catalog=1024, min_bytes=4096, max_bytes=64 << 20, rank_lo=2, rank_hi=6, elems=(4, 8, 16))
+    kw = dict(catalog=1024, min_bytes=4096, max_bytes=64 << 20, rank_lo=2, rank_hi=6, elems=(4, 8, 16),
+              inputs_in_arena=True)
     # Capacity: a rate far above what the device sustains (closed-loop saturation).
     srv.run_poisson(pool.ptr, pool_bytes, qps=1e7, requests=2000, seed=1, **kw)  # warm-up
     srv.reset_stats()
     sat = srv.run_poisson(pool.ptr, pool_bytes, qps=1e7, requests=args.requests, seed=2, **kw)
     mean_req_bytes = sat["bytes"] / max(1, sat["completed"])
     cap_qps = sat["gbs"] * 1e9 / mean_req_bytes
-    log("saturation: %.1f GB/s, %.0f req/s, mean request %.2f MB moved, mean batch %.1f" % (
+    log("saturation (warm plans): %.1f GB/s, %.0f req/s, mean request %.2f MB moved, mean batch %.1f" % (
         sat["gbs"], cap_qps, mean_req_bytes / 1e6, sat["mean_batch"]))
     rates = []
     for frac in (0.25, 0.5, 0.75, 0.9):
@@ -538,29 +656,44 @@ def bench_serving(args, stream):
                           arena_peak_gb=round(st["arena_peak_bytes"] / 1e9, 2)))
         log("load %.2f: %8.0f req/s  %7.1f GB/s  p50 %8.1f us  p99 %8.1f us  batch %.1f  stalls %d" % (
             frac, frac * cap_qps, st["gbs"], st["p50_us"], st["p99_us"], st["mean_batch"], st["admission_stalls"]))
+    # New shape per request: plans are made on arrival by 8 client threads.
+    fresh = dict(kw, fresh=True, clients=8)
+    srv.reset_stats()
+    fsat = srv.run_poisson(pool.ptr, pool_bytes, qps=1e7, requests=args.requests, seed=5, **fresh)
+    fcap = fsat["gbs"] * 1e9 / max(1.0, fsat["bytes"] / max(1, fsat["completed"]))
+    srv.reset_stats()
+    fhalf = srv.run_poisson(pool.ptr, pool_bytes, qps=0.5 * fcap, requests=args.requests, seed=6, **fresh)
+    plan_us = 1e6 * fsat["plan_seconds"] / max(1, fsat["plan_misses"])
+    log("fresh shapes: saturation %.1f GB/s (%.0f req/s), cold plan %.0f us per shape; at half that rate p50 %.1f us "
+        "p99 %.1f us" % (fsat["gbs"], fcap, plan_us, fhalf["p50_us"], fhalf["p99_us"]))
+    checked = serving_checked_pass(srv, stream, 300, 8)
     srv.destroy()
     # Per-call baseline: the same catalog through librettExecute, one stream,
     # plans built beforehand, back to back.
-    import ctypes
     cat = serving_catalog(kw, 1024)
     handles = []
     for d, p, e in cat:
         r, h = L.librettPlan(len(d), list(d), list(p), e, stream)
+        if r != L.LIBRETT_SUCCESS:
+            raise SystemExit("plan failed for %s %s" % (d, p))
         handles.append((h, int(np.prod(d)) * e))
     out = hip.DeviceBuffer(64 << 20)
     rng = np.random.default_rng(4)
     picks = rng.integers(0, len(handles), size=args.requests)
     offs = rng.integers(0, (pool_bytes - (64 << 20)) // 256, size=args.requests) * 256
-    for i in range(200):
+
+    def call(i):
         h, nb = handles[picks[i]]
-        L.librettExecute(h, pool.ptr + int(offs[i]), out.ptr)
+        if L.librettExecute(h, pool.ptr + int(offs[i]), out.ptr) != L.LIBRETT_SUCCESS:
+            raise SystemExit("librettExecute failed in the per-call loop")
+        return 2 * nb
+    for i in range(200):
+        call(i)
     hip.stream_sync(stream)
     t0 = time.perf_counter()
     moved = 0
     for i in range(args.requests):
-        h, nb = handles[picks[i]]
-        L.librettExecute(h, pool.ptr + int(offs[i]), out.ptr)
-        moved += 2 * nb
+        moved += call(i)
     hip.stream_sync(stream)
     loop_gbs = moved / (time.perf_counter() - t0) / 1e9
     log("per-call loop: %.1f GB/s" % loop_gbs)
@@ -570,17 +703,14 @@ def bench_serving(args, stream):
     t0 = time.perf_counter()
     moved = 0
     for i in range(args.requests):
-        h, nb = handles[picks[i]]
-        L.librettExecute(h, pool.ptr + int(offs[i]), out.ptr)
+        moved += call(i)
         hip.stream_sync(stream)
-        moved += 2 * nb
     sync_gbs = moved / (time.perf_counter() - t0) / 1e9
     log("per-call loop, synchronised per request: %.1f GB/s" % sync_gbs)
     for h, _ in handles:
         L.librettDestroy(h)
     out.free()
     pool.free()
-    best = max(rates, key=lambda r: r["gbs"])
     line = {
         "metric": "serving transpose GB/s and p50/p99 latency (Poisson arrivals, continuous batching)",
         "value": round(sat["gbs"], 1),
@@ -595,17 +725,31 @@ def bench_serving(args, stream):
         "dtype": "f32/f64/c128 (opaque moves)",
         "data": "synthetic",
         "config": {"workload": "serving: Poisson arrivals, 4 KiB-64 MiB log-uniform, rank 2-6, f32/f64/c128, "
-                               "%g GiB arena, LRU plan cache, catalog of 1024 shapes" % args.arena_gb,
+                               "%g GiB arena gating input + output bytes, LRU plan cache" % args.arena_gb,
                    "slots": args.slots, "max_batch": args.max_batch},
         "saturation": {"gbs": round(sat["gbs"], 1), "req_per_s": round(cap_qps), "p50_us": round(sat["p50_us"], 1),
-                       "p99_us": round(sat["p99_us"], 1), "mean_batch": round(sat["mean_batch"], 2)},
+                       "p99_us": round(sat["p99_us"], 1), "mean_batch": round(sat["mean_batch"], 2),
+                       "shapes": "catalog of 1024, planned up front"},
         "load_sweep": rates,
+        "fresh_shapes": {"saturation_gbs": round(fsat["gbs"], 1), "req_per_s": round(fcap),
+                         "cold_plan_us_mean": round(plan_us, 1), "plan_misses": fsat["plan_misses"],
+                         "client_threads": 8, "half_load_p50_us": round(fhalf["p50_us"], 1),
+                         "half_load_p99_us": round(fhalf["p99_us"], 1), "half_load_gbs": round(fhalf["gbs"], 1)},
         "per_call_loop_gbs": round(loop_gbs, 1),
         "speedup_vs_per_call": round(sat["gbs"] / loop_gbs, 2),
         "per_call_synchronised_gbs": round(sync_gbs, 1),
         "speedup_vs_per_call_synchronised": round(sat["gbs"] / sync_gbs, 2),
+        "checked_requests": checked,
     }
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def summarize_serving(r):
+    out = {k: r[k] for k in ("saturation", "load_sweep", "fresh_shapes", "per_call_loop_gbs", "speedup_vs_per_call",
+                             "per_call_synchronised_gbs", "speedup_vs_per_call_synchronised")}
+    out["workload"] = r["config"]["workload"]
+    out["checked"] = "%d requests of the same distribution checked on the device" % r["checked_requests"]
+    return out
 
 
 def serving_catalog(kw, n):

@@ -249,17 +249,23 @@ typedef struct librett_c_server_stats {
   double p50_us, p90_us, p99_us, max_us, mean_us; /* arrival -> completion */
   double wall_seconds;          /* first arrival -> last completion */
   double gbs;                   /* bytes / wall_seconds */
+  long long plan_misses;        /* requests whose shape was not in the plan cache */
+  double plan_seconds;          /* time spent planning those shapes (in the submitting threads) */
 } librett_c_server_stats;
 
 typedef struct librett_c_poisson {
-  double qps;                 /* mean arrival rate */
+  double qps;                 /* mean arrival rate (all clients together) */
   int requests;               /* number of arrivals */
   unsigned long long seed;
-  int catalog;                /* distinct request shapes, drawn up front */
+  int catalog;                /* distinct request shapes, drawn and planned up front */
   double min_bytes, max_bytes;/* tensor bytes, log-uniform */
   int rank_lo, rank_hi;
   int elems[3];               /* element sizes drawn uniformly */
   int nelems;
+  int fresh;                  /* != 0: every request draws a new shape (no catalog), planned when it arrives */
+  int inputs_in_arena;        /* != 0: inputs are allocated from the arena too, so admission waits for room for
+                                 input and output; their contents are left as the arena holds them */
+  int clients;                /* arrival threads, each a Poisson stream of qps / clients (<= 1: one) */
 } librett_c_poisson;
 
 int librett_c_server_create(unsigned int* handle, size_t arena_bytes, int slots, int max_batch);
@@ -278,10 +284,12 @@ int librett_c_server_stats_get(unsigned int handle, librett_c_server_stats* stat
 int librett_c_server_reset_stats(unsigned int handle);
 int librett_c_server_destroy(unsigned int handle);
 /* Native Poisson load generator: `requests` arrivals at `qps`, shapes from a
- * catalog, inputs read from slices of `input_pool` (device memory of
- * pool_bytes >= max_bytes), outputs allocated from the arena at arrival
- * (waiting while it is full) and returned at completion. Blocks until all
- * requests complete. */
+ * catalog planned up front or (fresh != 0) a new shape per request planned on
+ * arrival by the arriving client thread, inputs read from slices of
+ * `input_pool` (device memory of pool_bytes >= max_bytes) or taken from the
+ * arena (inputs_in_arena != 0), outputs allocated from the arena at arrival
+ * (waiting while it is full) and everything returned at completion. Blocks
+ * until all requests complete. */
 int librett_c_server_run_poisson(unsigned int handle, const librett_c_poisson* cfg, const void* input_pool,
                                  size_t pool_bytes);
 

@@ -105,7 +105,7 @@ class ServerStats(ctypes.Structure):
         ("arena_peak_bytes", ctypes.c_longlong), ("arena_in_use_bytes", ctypes.c_longlong),
         ("p50_us", ctypes.c_double), ("p90_us", ctypes.c_double), ("p99_us", ctypes.c_double),
         ("max_us", ctypes.c_double), ("mean_us", ctypes.c_double), ("wall_seconds", ctypes.c_double),
-        ("gbs", ctypes.c_double),
+        ("gbs", ctypes.c_double), ("plan_misses", ctypes.c_longlong), ("plan_seconds", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -116,7 +116,8 @@ class PoissonConfig(ctypes.Structure):
     _fields_ = [
         ("qps", ctypes.c_double), ("requests", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("catalog", ctypes.c_int),
         ("min_bytes", ctypes.c_double), ("max_bytes", ctypes.c_double), ("rank_lo", ctypes.c_int),
-        ("rank_hi", ctypes.c_int), ("elems", ctypes.c_int * 3), ("nelems", ctypes.c_int),
+        ("rank_hi", ctypes.c_int), ("elems", ctypes.c_int * 3), ("nelems", ctypes.c_int), ("fresh", ctypes.c_int),
+        ("inputs_in_arena", ctypes.c_int), ("clients", ctypes.c_int),
     ]
 
 
@@ -477,10 +478,11 @@ class Server:
         lib().librett_c_server_reset_stats(self.handle)
 
     def run_poisson(self, pool_ptr, pool_bytes, qps, requests, seed=20261202, catalog=1024, min_bytes=4096,
-                    max_bytes=64 << 20, rank_lo=2, rank_hi=6, elems=(4, 8, 16)):
+                    max_bytes=64 << 20, rank_lo=2, rank_hi=6, elems=(4, 8, 16), fresh=False, inputs_in_arena=False,
+                    clients=1):
         cfg = PoissonConfig(float(qps), int(requests), int(seed), int(catalog), float(min_bytes), float(max_bytes),
                             int(rank_lo), int(rank_hi), (ctypes.c_int * 3)(*(list(elems) + [4, 4, 4])[:3]),
-                            len(elems))
+                            len(elems), int(bool(fresh)), int(bool(inputs_in_arena)), int(clients))
         r = lib().librett_c_server_run_poisson(self.handle, ctypes.byref(cfg), ctypes.c_void_p(pool_ptr),
                                                int(pool_bytes))
         if r != LIBRETT_SUCCESS:

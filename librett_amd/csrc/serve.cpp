@@ -139,6 +139,7 @@ struct Request {
   const void* in = nullptr;
   void* out = nullptr;
   bool release_out = false;  // generator requests return their output to the arena
+  bool release_in = false;   // ... and their input, when it came from the arena too
   double t_arrival = 0, t_done = 0;
   int status = 0;            // 0 queued, 1 running, 2 done, <0 failed
 };
@@ -221,25 +222,32 @@ class Server {
 
   Arena& arena() { return arena_; }
 
-  // Plan of a problem from the LRU cache, planned on a miss.
+  // Plan of a problem from the LRU cache, planned on a miss (in the calling
+  // thread; the time misses take is part of the statistics).
   bool plan_for(const Problem& p, Candidate& c) {
     const std::string key = key_of(p);
     if (plans_.get(key, c)) return true;
+    const double t0 = now_s();
     std::vector<Candidate> cands = batch_candidates(p, dev_);
     if (cands.empty()) return false;
     c = cands[0];
     plans_.put(key, c);
+    const double dt = now_s() - t0;
+    std::lock_guard<std::mutex> lock(mu_);
+    plan_misses_++;
+    plan_s_ += dt;
     return true;
   }
 
   int submit(const Problem& p, const void* in, void* out, bool release_out, double t_arrival,
-             unsigned long long* ticket) {
+             unsigned long long* ticket, bool release_in = false) {
     auto r = std::make_shared<Request>();
     r->prob = p;
     if (!plan_for(p, r->plan)) return LIBRETT_C_INTERNAL_ERROR;
     r->in = in;
     r->out = out;
     r->release_out = release_out;
+    r->release_in = release_in;
     r->t_arrival = t_arrival > 0 ? t_arrival : now_s();
     {
       std::lock_guard<std::mutex> lock(mu_);
@@ -300,6 +308,8 @@ class Server {
       st->wall_seconds = last_done_ - first_arrival_;
       st->gbs = st->wall_seconds > 0 ? (double)bytes_ / st->wall_seconds / 1e9 : 0;
     }
+    st->plan_misses = plan_misses_;
+    st->plan_seconds = plan_s_;
   }
 
   void reset_stats() {
@@ -310,6 +320,8 @@ class Server {
     stalls_ = 0;
     stall_s_ = 0;
     submitted_ = failed_ = 0;
+    plan_misses_ = 0;
+    plan_s_ = 0;
     first_arrival_ = 1e300;
     last_done_ = 0;
     finished_.clear();
@@ -418,6 +430,7 @@ class Server {
       first_arrival_ = std::min(first_arrival_, r->t_arrival);
       last_done_ = std::max(last_done_, t);
       if (r->release_out) arena_.release(r->out);
+      if (r->release_in) arena_.release(const_cast<void*>(r->in));
       // Latencies of finished requests stay available to wait() for the
       // most recent kFinishedKeep of them.
       finished_[r->id] = t - r->t_arrival;
@@ -494,7 +507,8 @@ class Server {
   bool stop_ = true;
   unsigned long long next_id_ = 1, completions_ = 0;
   long long submitted_ = 0, failed_ = 0, bytes_ = 0, batches_ = 0, batched_reqs_ = 0, stalls_ = 0;
-  double stall_s_ = 0, first_arrival_ = 1e300, last_done_ = 0;
+  long long plan_misses_ = 0;
+  double stall_s_ = 0, plan_s_ = 0, first_arrival_ = 1e300, last_done_ = 0;
   std::vector<double> lat_;
 };
 
@@ -649,55 +663,106 @@ int librett_c_server_run_poisson(unsigned int handle, const librett_c_poisson* c
                                  size_t pool_bytes) {
   auto s = find_server(handle);
   if (!s) return LIBRETT_C_INVALID_PLAN;
-  if (!cfg || !input_pool || cfg->qps <= 0 || cfg->requests < 1 || cfg->catalog < 1 || cfg->min_bytes <= 0 ||
-      cfg->max_bytes < cfg->min_bytes || cfg->rank_lo < 1 || cfg->rank_hi < cfg->rank_lo || cfg->nelems < 1 ||
-      (size_t)cfg->max_bytes > pool_bytes)
+  if (!cfg || !input_pool || cfg->qps <= 0 || cfg->requests < 1 || (!cfg->fresh && cfg->catalog < 1) ||
+      cfg->min_bytes <= 0 || cfg->max_bytes < cfg->min_bytes || cfg->rank_lo < 1 || cfg->rank_hi < cfg->rank_lo ||
+      cfg->nelems < 1 || cfg->nelems > 3 || (size_t)cfg->max_bytes > pool_bytes)
     return LIBRETT_C_INVALID_PARAMETER;
+  for (int i = 0; i < cfg->nelems; i++)
+    if (cfg->elems[i] != 4 && cfg->elems[i] != 8 && cfg->elems[i] != 16) return LIBRETT_C_INVALID_PARAMETER;
   try {
-    std::mt19937_64 g(cfg->seed);
-    // Catalog of request shapes (bytes log-uniform in [min, max]), planned
-    // up front so the plan cache is warm.
-    std::vector<Problem> catalog;
+    // A request shape of log-uniform bytes in [min, max]; false if it does
+    // not fit the pool or the arena (the caller draws again).
     std::uniform_real_distribution<double> lu(std::log(cfg->min_bytes), std::log(cfg->max_bytes));
     std::uniform_int_distribution<int> pe(0, cfg->nelems - 1);
-    for (int i = 0; i < cfg->catalog; i++) {
+    const size_t room = cfg->inputs_in_arena ? s->arena().capacity() / 2 : s->arena().capacity();
+    auto draw = [&](std::mt19937_64& g, Problem& p) {
       const int elem = cfg->elems[pe(g)];
       std::vector<int> dim, perm;
       random_problem(g, std::exp(lu(g)), cfg->rank_lo, cfg->rank_hi, elem, dim, perm);
-      if (check_problem((int)dim.size(), dim.data(), perm.data(), elem) != LIBRETT_C_SUCCESS) continue;
-      Problem p = canonicalize((int)dim.size(), dim.data(), perm.data(), elem);
-      if ((size_t)p.vol * p.elem > pool_bytes || (size_t)p.vol * p.elem > s->arena().capacity()) continue;
-      Candidate c;
-      if (!s->plan_for(p, c)) continue;
-      catalog.push_back(p);
-    }
-    if (catalog.empty()) return LIBRETT_C_INTERNAL_ERROR;
-    std::exponential_distribution<double> gap(cfg->qps);
-    std::uniform_int_distribution<size_t> pick(0, catalog.size() - 1);
-    const double t0 = now_s();
-    double t_next = t0;
-    for (int i = 0; i < cfg->requests; i++) {
-      t_next += gap(g);
-      const Problem& p = catalog[pick(g)];
+      if (check_problem((int)dim.size(), dim.data(), perm.data(), elem) != LIBRETT_C_SUCCESS) return false;
+      p = canonicalize((int)dim.size(), dim.data(), perm.data(), elem);
       const size_t bytes = (size_t)p.vol * p.elem;
-      // Input: a 256-byte-aligned slice of the resident pool.
-      const size_t slots = (pool_bytes - bytes) / 256 + 1;
-      const char* in = (const char*)input_pool + (std::uniform_int_distribution<size_t>(0, slots - 1)(g)) * 256;
-      while (now_s() < t_next) std::this_thread::yield();
-      void* out = s->arena().alloc(bytes);
-      if (!out) {
-        const double ts = now_s();
-        while (!(out = s->arena().alloc(bytes))) s->wait_any_completion(1e-3);
-        s->add_stall(now_s() - ts);
+      return bytes <= pool_bytes && bytes <= room;
+    };
+    // Catalog of request shapes, planned up front so the plan cache is warm.
+    std::vector<Problem> catalog;
+    if (!cfg->fresh) {
+      std::mt19937_64 g(cfg->seed);
+      for (int i = 0; i < cfg->catalog; i++) {
+        Problem p;
+        Candidate c;
+        if (draw(g, p) && s->plan_for(p, c)) catalog.push_back(p);
       }
-      const int r = s->submit(p, in, out, true, t_next, nullptr);
-      if (r != LIBRETT_C_SUCCESS) {
-        s->arena().release(out);
-        return r;
+      if (catalog.empty()) return LIBRETT_C_INTERNAL_ERROR;
+    }
+    const int clients = std::max(1, std::min(cfg->clients, 64));
+    std::atomic<int> failed_rc{LIBRETT_C_SUCCESS};
+    const double t0 = now_s();
+    // Client k: its own Poisson stream of rate qps / clients and its share of
+    // the requests; shapes from the catalog or drawn fresh (then planned in
+    // submit, on this thread).
+    auto client = [&](int k) {
+      std::mt19937_64 g(cfg->seed * 7919 + 104729ULL * (k + 1));
+      std::exponential_distribution<double> gap(cfg->qps / clients);
+      const int n = cfg->requests / clients + (k < cfg->requests % clients ? 1 : 0);
+      double t_next = t0;
+      for (int i = 0; i < n && failed_rc.load() == LIBRETT_C_SUCCESS; i++) {
+        t_next += gap(g);
+        Problem p;
+        if (cfg->fresh) {
+          int tries = 0;
+          while (!draw(g, p))
+            if (++tries > 1000) {
+              failed_rc = LIBRETT_C_INTERNAL_ERROR;
+              return;
+            }
+        } else {
+          p = catalog[std::uniform_int_distribution<size_t>(0, catalog.size() - 1)(g)];
+        }
+        const size_t bytes = (size_t)p.vol * p.elem;
+        // Pool slice for the input unless it comes from the arena.
+        const size_t nslices = (pool_bytes - bytes) / 256 + 1;
+        const char* pool_in =
+            (const char*)input_pool + (std::uniform_int_distribution<size_t>(0, nslices - 1)(g)) * 256;
+        while (now_s() < t_next) std::this_thread::yield();
+        // Admission: room for the output (and the input) in the arena, all
+        // or nothing, so waiting clients hold no partial reservations.
+        void* out = nullptr;
+        void* in = nullptr;
+        auto admit = [&] {
+          out = s->arena().alloc(bytes);
+          if (!out) return false;
+          if (!cfg->inputs_in_arena) return true;
+          in = s->arena().alloc(bytes);
+          if (in) return true;
+          s->arena().release(out);
+          out = nullptr;
+          return false;
+        };
+        if (!admit()) {
+          const double ts = now_s();
+          while (!admit()) s->wait_any_completion(1e-3);
+          s->add_stall(now_s() - ts);
+        }
+        const int r = s->submit(p, cfg->inputs_in_arena ? in : pool_in, out, true, t_next, nullptr,
+                                cfg->inputs_in_arena != 0);
+        if (r != LIBRETT_C_SUCCESS) {
+          s->arena().release(out);
+          if (in) s->arena().release(in);
+          failed_rc = r;
+          return;
+        }
       }
+    };
+    if (clients == 1) {
+      client(0);
+    } else {
+      std::vector<std::thread> ts;
+      for (int k = 0; k < clients; k++) ts.emplace_back(client, k);
+      for (auto& t : ts) t.join();
     }
     s->drain();
-    return LIBRETT_C_SUCCESS;
+    return failed_rc.load();
   } catch (...) {
     return LIBRETT_C_INTERNAL_ERROR;
   }

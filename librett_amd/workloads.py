@@ -116,3 +116,11 @@ def test2_dims(rng, rank, volmax=2e8, min_dim=2.0):
         if not (curvol > volmax or abs(curvol - vol) / vol > 2.3):
             return list(dims)
     raise RuntimeError("no dims found for rank %d" % rank)
+
+
+def batched_problems(n=4096, seed=20261202):
+    """The batched workload: `n` problems of rank 3-5, volume log-uniform in
+    [1e3, 1e6] elements, float64 or complex128, random non-identity
+    permutations."""
+    rng = np.random.default_rng(seed)
+    return [random_problem(rng, 3, 5, 1e3, 1e6, [8, 16]) for _ in range(n)]

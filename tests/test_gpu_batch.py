@@ -185,3 +185,36 @@ def test_autotuner_not_slower_than_model(gpu_stream):
     say("  model %.3f ms, autotuned %.3f ms" % (times[0], times[3]))
     assert times[3] <= 1.05 * times[0]
     slab.free()
+
+
+def test_config3_full_batch_checked_on_device(gpu_stream):
+    # The batched workload at its stated size: 4096 problems of 1e3-1e6
+    # elements (f64 / c128), autotuned, one graph launch; every output
+    # checked on the device against the index pattern of its input.
+    probs = W.batched_problems(4096)
+    sizes = [int(np.prod(d)) * e for d, p, e in probs]
+    offs, o = [], 0
+    for s_ in sizes:
+        offs.append(o)
+        o += (s_ + 255) // 256 * 256
+    say("case: 4096-problem batch, %.2f GB per side" % (o / 1e9))
+    A, B = hip.DeviceBuffer(o), hip.DeviceBuffer(o)
+    try:
+        for off, s_ in zip(offs, sizes):
+            L.check_fill(A.ptr + off, s_, gpu_stream)
+        hip.memset(B.ptr, 0xEE, o)
+        hip.stream_sync(gpu_stream)
+        b = L.Batch(probs, gpu_stream, tune=3)
+        say("  info", b.info())
+        b.set_buffers([A.ptr + x for x in offs], [B.ptr + x for x in offs])
+        for rep in range(2):
+            assert b.execute(use_graph=True) == L.LIBRETT_SUCCESS
+            hip.stream_sync(gpu_stream)
+            bad = [i for i, ((d, p, e), off) in enumerate(zip(probs, offs))
+                   if L.check_transpose(B.ptr + off, d, p, e, gpu_stream)[0]]
+            assert not bad, (rep, [probs[i] for i in bad[:5]])
+            hip.memset(B.ptr, 0xEE, o)
+        b.destroy()
+    finally:
+        A.free()
+        B.free()

@@ -75,3 +75,58 @@ def test_small_arena_admission_and_accounting():
     assert srv.alloc(budget + 4096) is None  # larger than the arena: refused
     srv.destroy()
     pool.free()
+
+
+def test_full_size_requests_checked_on_device(gpu_stream):
+    # Config-4 sizes: 4 KiB - 64 MiB log-uniform, rank 2-6, f32/f64/c128,
+    # inputs and outputs from the arena; every output checked on the device.
+    rng = np.random.default_rng(44)
+    srv = L.Server(16 << 30, slots=8, max_batch=64)
+    reqs = []
+    for _ in range(240):
+        e = int(rng.choice([4, 8, 16]))
+        nb = float(np.exp(rng.uniform(np.log(4096), np.log(64 << 20))))
+        r = int(rng.integers(2, 7))
+        reqs.append(W.random_problem(rng, r, r, max(2.0, nb / e), max(2.0, nb / e), [e]))
+    reqs.append(([2048, 2048, 4], [1, 0, 2], 4))  # exactly 64 MiB
+    say("case: %d full-size serving requests, largest %.1f MiB" % (
+        len(reqs), max(np.prod(d) * e for d, _, e in reqs) / 2 ** 20))
+    for wave in range(0, len(reqs), 80):
+        part = reqs[wave:wave + 80]
+        bufs = []
+        for d, p, e in part:
+            n = int(np.prod(d)) * e
+            pin, pout = srv.alloc(n), srv.alloc(n)
+            assert pin is not None and pout is not None
+            L.check_fill(pin, n, gpu_stream)
+            bufs.append((pin, pout))
+        hip.stream_sync(gpu_stream)
+        tickets = [srv.submit(d, p, e, pin, pout) for (d, p, e), (pin, pout) in zip(part, bufs)]
+        for t in tickets:
+            srv.wait(t)
+        for (d, p, e), (pin, pout) in zip(part, bufs):
+            assert L.check_transpose(pout, d, p, e, gpu_stream) == (0, -1), (d, p, e)
+            assert srv.free(pin) == L.LIBRETT_SUCCESS and srv.free(pout) == L.LIBRETT_SUCCESS
+    st = srv.stats()
+    assert st["failed"] == 0 and st["completed"] == len(reqs)
+    assert st["arena_in_use_bytes"] == 0
+    srv.destroy()
+
+
+def test_poisson_fresh_shapes_inputs_in_arena():
+    # Every request a new shape (planned on arrival by one of four client
+    # threads); inputs and outputs both reserved in the arena.
+    say("case: Poisson, fresh shapes, inputs in the arena, 4 clients")
+    srv = L.Server(4 << 30, slots=8, max_batch=64)
+    pool = hip.DeviceBuffer(64 << 20)
+    st = srv.run_poisson(pool.ptr, 64 << 20, qps=20000, requests=2000, seed=12, min_bytes=4096,
+                         max_bytes=64 << 20, fresh=True, inputs_in_arena=True, clients=4)
+    say("  stats", {k: st[k] for k in ("completed", "plan_misses", "plan_seconds", "p50_us", "p99_us", "gbs",
+                                        "arena_peak_bytes")})
+    assert st["completed"] == 2000 and st["failed"] == 0
+    assert st["plan_misses"] >= 1900  # random shapes rarely repeat
+    assert st["plan_seconds"] > 0
+    assert st["arena_in_use_bytes"] == 0
+    assert st["arena_peak_bytes"] <= 4 << 30
+    srv.destroy()
+    pool.free()
