@@ -395,6 +395,21 @@ int librett_c_check_fill(void* data, size_t nbytes, void* stream);
 int librett_c_check_transpose(const void* odata, int rank, const int* dim, const int* permutation, size_t sizeofType,
                               void* stream, unsigned long long* mismatches, long long* first);
 void librett_c_host_check_fill(void* data, size_t nbytes);
+
+/* Allocation-free variant of librett_c_check_transpose. It has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). It allocates nothing and does not synchronise: it
+ * enqueues the check on `stream` and writes result[0] = wrong elements and
+ * result[1] = first wrong output position (all ones if none) to `result`,
+ * two unsigned long longs of device memory owned by the caller. Its scratch
+ * is a fixed 64 KiB array in device memory per GPU (4096 per-workgroup
+ * partials), kept until the process ends; problems whose check would need
+ * more partials are refused with LIBRETT_C_INVALID_PARAMETER (the checker's
+ * grid is capped at 4096, so none are today). Calls that overlap on one GPU
+ * share that scratch and race: serialise them (one stream, or events). */
+int librett_c_check_transpose_noalloc(const void* odata, int rank, const int* dim, const int* permutation,
+                                      size_t sizeofType, void* stream, unsigned long long* result);
 int librett_c_host_check_transpose(const void* odata, int rank, const int* dim, const int* permutation,
                                    size_t sizeofType, unsigned long long* mismatches, long long* first);
 

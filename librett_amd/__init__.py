@@ -81,6 +81,7 @@ EXPORTED = [
     "librett_c_mlp_grad", "librett_c_mlp_adam", "librett_c_mlp_train", "librett_c_mlp_host_forward",
     "librett_c_mlp_host_grad", "librett_c_mlp_host_adam",
     "librett_c_check_fill", "librett_c_check_transpose", "librett_c_host_check_fill", "librett_c_host_check_transpose",
+    "librett_c_check_transpose_noalloc",
     "librett_c_set_allocator", "librett_c_allocator_live",
     "librett_c_plan_counters", "librett_c_candidate_counters", "librett_c_host_candidate_counters",
     "librett_c_host_access_trace", "librett_c_selector_margin",
@@ -235,6 +236,7 @@ def lib():
         L.librett_c_check_fill.argtypes = [vp, ctypes.c_size_t, vp]
         L.librett_c_check_transpose.argtypes = [vp, ctypes.c_int, ip, ip, ctypes.c_size_t, vp, ullp, llp]
         L.librett_c_host_check_fill.argtypes = [vp, ctypes.c_size_t]
+        L.librett_c_check_transpose_noalloc.argtypes = [vp, ctypes.c_int, ip, ip, ctypes.c_size_t, vp, vp]
         L.librett_c_host_check_transpose.argtypes = [vp, ctypes.c_int, ip, ip, ctypes.c_size_t, ullp, llp]
         L.librett_c_set_allocator.argtypes = [ALLOC_FN, FREE_FN, vp]
         L.librett_c_allocator_live.argtypes = []
@@ -693,6 +695,14 @@ def check_transpose(optr, dim, permutation, sizeofType, stream):
     if r != LIBRETT_SUCCESS:
         raise ValueError("check_transpose failed: %d" % r)
     return n.value, f.value
+
+
+def check_transpose_noalloc(optr, dim, permutation, sizeofType, stream, result_ptr):
+    """Enqueues the allocation-free check; the device result (2 x uint64 at
+    result_ptr) is ready when `stream` is."""
+    return lib().librett_c_check_transpose_noalloc(ctypes.c_void_p(optr), len(dim), _iarr(dim),
+                                                   _iarr(permutation), int(sizeofType), ctypes.c_void_p(stream),
+                                                   ctypes.c_void_p(result_ptr))
 
 
 def host_check_fill(arr):

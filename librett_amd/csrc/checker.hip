@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "devmem.h"
@@ -69,7 +70,65 @@ __global__ __launch_bounds__(256) void k_check(CheckArgs a, const uint32_t* __re
   }
 }
 
+// Sums the per-workgroup partials of k_check into result[0] (wrong
+// elements) and result[1] (first wrong output position, ~0 if none).
+__global__ __launch_bounds__(256) void k_check_reduce(const CheckPartial* __restrict__ part, int n,
+                                                      unsigned long long* __restrict__ result) {
+  __shared__ unsigned long long cnt[256];
+  __shared__ uint32_t first[256];
+  unsigned long long c = 0;
+  uint32_t f = 0xffffffffu;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    c += part[i].mismatches;
+    f = min(f, part[i].first);
+  }
+  cnt[threadIdx.x] = c;
+  first[threadIdx.x] = f;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      cnt[threadIdx.x] += cnt[threadIdx.x + s];
+      first[threadIdx.x] = min(first[threadIdx.x], first[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    result[0] = cnt[0];
+    result[1] = cnt[0] ? (unsigned long long)first[0] : ~0ull;
+  }
+}
+
 }  // namespace
+
+// Scratch of the allocation-free checker: one partial per workgroup of
+// k_check (whose grid never exceeds kCheckMaxGrid), 64 KiB. Every GPU the
+// library runs on holds its own copy until the process ends. Host code
+// never passes this name to a kernel or a copy: it uses the device address
+// from hipGetSymbolAddress (scratch_address below).
+static __device__ __align__(256) CheckPartial scratch[kCheckMaxGrid];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept; nullptr if the lookup fails.
+static CheckPartial* scratch_address() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static CheckPartial* addr[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    addr[dev] = static_cast<CheckPartial*>(p);
+  }
+  return addr[dev];
+}
 
 bool make_check_args(int rank, const int* dim, const int* perm, int elem, CheckArgs& a) {
   if (rank < 1 || rank > kMaxRank || (elem != 4 && elem != 8 && elem != 16)) return false;
@@ -172,6 +231,33 @@ int librett_c_check_transpose(const void* odata, int rank, const int* dim, const
   }
   *mismatches = n;
   if (first) *first = n ? (long long)f : -1;
+  return LIBRETT_C_SUCCESS;
+}
+
+int librett_c_check_transpose_noalloc(const void* odata, int rank, const int* dim, const int* permutation,
+                                      size_t sizeofType, void* stream, unsigned long long* result) {
+  if (!odata || !dim || !permutation || !result) return LIBRETT_C_INVALID_PARAMETER;
+  CheckArgs a;
+  if (!make_check_args(rank, dim, permutation, (int)sizeofType, a)) return LIBRETT_C_INVALID_PARAMETER;
+  const int grid = check_grid(a);
+  // The scratch holds kCheckMaxGrid partials; check_grid never asks for more.
+  if (grid < 1 || grid > kCheckMaxGrid) return LIBRETT_C_INVALID_PARAMETER;
+  CheckPartial* part = scratch_address();
+  if (!part) return LIBRETT_C_INTERNAL_ERROR;
+  hipStream_t s = (hipStream_t)stream;
+  // Every workgroup of k_check writes its own slot, and the reduction reads
+  // only those slots, so nothing left from an earlier call is read.
+  if (launch_check(a, odata, part, grid, s) != hipSuccess) {
+    (void)hipGetLastError();
+    return LIBRETT_C_INTERNAL_ERROR;
+  }
+  const CheckPartial* cpart = part;
+  int n = grid;
+  void* args[3] = {&cpart, &n, &result};
+  if (hipLaunchKernel(reinterpret_cast<const void*>(&k_check_reduce), dim3(1), dim3(256), args, 0, s) != hipSuccess) {
+    (void)hipGetLastError();
+    return LIBRETT_C_INTERNAL_ERROR;
+  }
   return LIBRETT_C_SUCCESS;
 }
 
