@@ -89,12 +89,42 @@ __host__ __device__ inline void outer_offsets(const OuterMap& m, uint32_t k,
   off_out = oo;
 }
 
+// A group of dims walked as one linear index v of the tile plane: digit k of
+// v (extent of dim k, the first dim fastest) moves the other tensor by
+// stride[k] elements. A group of one dim is linear (n == 1: v * stride[0]).
+constexpr int kMaxGroup = 6;
+struct GroupMap {
+  int n;
+  FastDiv div[kMaxGroup - 1];  // extents of all but the last dim
+  int stride[kMaxGroup];
+};
+
+__host__ __device__ inline int group_offset(const GroupMap& g, uint32_t v) {
+  int off = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxGroup - 1; k++) {
+    if (k + 1 >= g.n) return off + (int)v * g.stride[k];
+    const uint32_t q = fdiv(v, g.div[k]);
+    off += (int)(v - q * g.div[k].d) * g.stride[k];
+    v = q;
+  }
+  return off + (int)v * g.stride[kMaxGroup - 1];
+}
+
 // Tiled: 2-D tile plane of (x = input dim 0, y = the input dim that becomes
 // output dim 0). Reference counterpart: transposeTiled, src/kernel.cpp:48-170.
+// x may also be a group of leading input dims (contiguous in the input, in
+// any order in the output) and y a group of leading output dims (contiguous
+// in the output, in any order in the input), so that tensors whose first
+// dims are short still get tiles with long runs on both sides; the output
+// offset of x (the input offset of y) is then group_offset(xg, x)
+// (group_offset(yg, y)) instead of x * out_xs (y * in_ys).
 struct TiledArgs {
   int X, Y;          // plane extents in elements
-  int in_ys;         // input stride of y
-  int out_xs;        // output stride of x
+  int in_ys;         // input stride of y (yg.n == 1)
+  int out_xs;        // output stride of x (xg.n == 1)
+  GroupMap xg;       // x digits -> output offset
+  GroupMap yg;       // y digits -> input offset
   int ntx;           // tiles along x
   int ntiles;        // tiles per plane
   FastDiv ntx_div;
@@ -111,6 +141,14 @@ struct TiledArgs {
   int last_group_start;   // first tile index of the last group
   OuterMap outer;
 };
+
+// Output offset of tile-plane column x and input offset of row y.
+__host__ __device__ inline int tiled_out_x(const TiledArgs& a, uint32_t x) {
+  return a.xg.n > 1 ? group_offset(a.xg, x) : (int)x * a.out_xs;
+}
+__host__ __device__ inline int tiled_in_y(const TiledArgs& a, uint32_t y) {
+  return a.yg.n > 1 ? group_offset(a.yg, y) : (int)y * a.in_ys;
+}
 
 // TiledCopy: rows that stay contiguous in input and output (perm[0] == 0),
 // moved as W-byte words. Plane = (x = words of dim 0, y = the input dim that

@@ -153,14 +153,14 @@ __host__ __device__ inline bool lane_access(const CountJob& j, const ItemCtx& c,
       const int x = c.x0 + (t % LR) * j.VX;
       const int y = c.y0 + t / LR + p * RP;
       if (x >= a.X || y >= a.Y) return false;
-      off = (long long)(c.in0 + x + (long long)y * a.in_ys) * j.B;
+      off = (long long)(c.in0 + x + (long long)tiled_in_y(a, (uint32_t)y)) * j.B;
     } else {
       const int LW = j.TY / j.VY;
       const int WP = kCountThreads / LW;
       const int y = c.y0 + (t % LW) * j.VY;
       const int x = c.x0 + t / LW + p * WP;
       if (x >= a.X || y >= a.Y) return false;
-      off = (long long)(c.out0 + y + (long long)x * a.out_xs) * j.B;
+      off = (long long)(c.out0 + y + (long long)tiled_out_x(a, (uint32_t)x)) * j.B;
     }
     return true;
   }
@@ -223,6 +223,19 @@ struct LineWalk {
   }
 };
 
+// Insertion sort of a few values.
+__host__ __device__ inline void sort_small(long long* v, int n) {
+  for (int i = 1; i < n; i++) {
+    const long long x = v[i];
+    int k = i - 1;
+    while (k >= 0 && v[k] > x) {
+      v[k + 1] = v[k];
+      k--;
+    }
+    v[k + 1] = x;
+  }
+}
+
 // Lines one item reads and writes: Tiled reads tile rows (y ascending) and
 // writes tile columns (x ascending); TiledCopy reads and writes rows; Packed
 // walks the staged shape's runs, which the host sorted by start.
@@ -231,8 +244,15 @@ __host__ __device__ inline void item_lines(const CountJob& j, const ItemCtx& c, 
   const long long B = j.B;
   if (j.method == kTiled) {
     const TiledArgs& a = j.a.tiled;
-    for (int y = 0; y < c.ty; y++) rd.add((c.in0 + c.x0 + (long long)(c.y0 + y) * a.in_ys) * B, c.tx * B);
-    for (int x = 0; x < c.tx; x++) wr.add((c.out0 + c.y0 + (long long)(c.x0 + x) * a.out_xs) * B, c.ty * B);
+    // Rows (columns) of a one-dim group are in address order; those of a
+    // group of several dims are sorted first (a tile has at most 128).
+    long long st[128];
+    for (int y = 0; y < c.ty; y++) st[y] = (c.in0 + c.x0 + (long long)tiled_in_y(a, (uint32_t)(c.y0 + y))) * B;
+    if (a.yg.n > 1) sort_small(st, c.ty);
+    for (int y = 0; y < c.ty; y++) rd.add(st[y], c.tx * B);
+    for (int x = 0; x < c.tx; x++) st[x] = (c.out0 + c.y0 + (long long)tiled_out_x(a, (uint32_t)(c.x0 + x))) * B;
+    if (a.xg.n > 1) sort_small(st, c.tx);
+    for (int x = 0; x < c.tx; x++) wr.add(st[x], c.ty * B);
   } else if (j.method == kTiledCopy) {
     const CopyArgs& a = j.a.copy;
     for (int y = 0; y < c.ty; y++) {

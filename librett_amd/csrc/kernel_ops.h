@@ -93,9 +93,14 @@ struct TileCtx {
 // (contiguous in the input) with VX elements per lane; writes walk y
 // (contiguous in the output) with VY elements per lane. One LDS row of pitch
 // TX + 1 per y keeps the column reads of the write phase spread over banks.
+// When x or y is a group of dims, each row (column) finds its offset on the
+// other side from the group's digits; a full tile of one-dim groups steps
+// through memory by a fixed stride instead.
 // Reference counterpart: transposeTiled, src/kernel.cpp:48-170.
 // ---------------------------------------------------------------------------
-template <int B, int TX, int TY, int VX, int VY>
+// G = false compiles only the one-dim-group path (the kernel then needs no
+// registers for group maps); G = true handles groups of several dims too.
+template <int B, int TX, int TY, int VX, int VY, bool G = false>
 struct TiledOps {
   using E = typename Word<B>::type;
   using VI = typename Word<B * VX>::type;
@@ -140,7 +145,7 @@ struct TiledOps {
     const int ly = t / LR;
     const int x = c.x0 + lx;
     VI r[NPR];
-    if (c.full) {
+    if (c.full && (!G || a.yg.n == 1)) {
       const E* p = in + (c.off_in + x + (c.y0 + ly) * a.in_ys);
       const int step = RP * a.in_ys;
 #pragma unroll
@@ -154,7 +159,8 @@ struct TiledOps {
 #pragma unroll
       for (int j = 0; j < NPR; j++) {
         const int y = c.y0 + ly + j * RP;
-        if (xin && y < a.Y) r[j] = ld(reinterpret_cast<const VI*>(in + (c.off_in + x + y * a.in_ys)));
+        const int yoff = G ? tiled_in_y(a, (uint32_t)y) : y * a.in_ys;
+        if (xin && y < a.Y) r[j] = ld(reinterpret_cast<const VI*>(in + (c.off_in + x + yoff)));
       }
 #pragma unroll
       for (int j = 0; j < NPR; j++) {
@@ -177,7 +183,7 @@ struct TiledOps {
     for (int j = 0; j < NPW; j++)
 #pragma unroll
       for (int e = 0; e < VY; e++) vset<E>(r[j], e, lds[(ly + e) * PITCH + lx + j * WP]);
-    if (c.full) {
+    if (c.full && (!G || a.xg.n == 1)) {
       E* p = out + (c.off_out + y + (c.x0 + lx) * a.out_xs);
       const int step = WP * a.out_xs;
 #pragma unroll
@@ -187,7 +193,8 @@ struct TiledOps {
 #pragma unroll
       for (int j = 0; j < NPW; j++) {
         const int x = c.x0 + lx + j * WP;
-        if (yin && x < a.X) st(reinterpret_cast<VO*>(out + (c.off_out + y + x * a.out_xs)), r[j]);
+        const int xoff = G ? tiled_out_x(a, (uint32_t)x) : x * a.out_xs;
+        if (yin && x < a.X) st(reinterpret_cast<VO*>(out + (c.off_out + y + xoff)), r[j]);
       }
     }
   }

@@ -25,7 +25,8 @@ int prepare_kernel(const Candidate& c);
 // per 256*k words.
 hipError_t launch_device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream, int k, int nt, int grid);
 
-bool tiled_shape_available(int elem, int tx, int ty, int vx, int vy);
+// general: a kernel for planes whose x or y is a group of several dims.
+bool tiled_shape_available(int elem, int tx, int ty, int vx, int vy, bool general = false);
 // Shapes the batched engine's persistent kernel is instantiated for (batch.hip).
 bool batch_tiled_available(int elem, int tx, int ty, int vx, int vy);
 bool batch_copy_available(int word, int tx);

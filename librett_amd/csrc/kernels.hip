@@ -39,10 +39,10 @@ namespace lrt {
 // Device kernels
 // ---------------------------------------------------------------------------
 
-template <int B, int TX, int TY, int VX, int VY>
+template <int B, int TX, int TY, int VX, int VY, bool G>
 __global__ __launch_bounds__(256) void k_tiled(TiledArgs a, const void* __restrict__ in,
                                                void* __restrict__ out) {
-  using Ops = TiledOps<B, TX, TY, VX, VY>;
+  using Ops = TiledOps<B, TX, TY, VX, VY, G>;
   using E = typename Ops::E;
   __shared__ E lds[Ops::LDS_ELEMS];
   const E* src = static_cast<const E*>(in);
@@ -182,9 +182,9 @@ static inline void replay_at(long long item, int t) {
   if (g_replay_observer) g_replay_observer->at(item, t);
 }
 
-template <int B, int TX, int TY, int VX, int VY>
+template <int B, int TX, int TY, int VX, int VY, bool G>
 static void replay_tiled(const Candidate& c, const void* in, void* out) {
-  using Ops = TiledOps<B, TX, TY, VX, VY>;
+  using Ops = TiledOps<B, TX, TY, VX, VY, G>;
   using E = typename Ops::E;
   std::vector<E> lds(Ops::LDS_ELEMS);
   const TiledArgs& a = c.tiled;
@@ -296,6 +296,7 @@ static void replay_packed(const Candidate& c, const void* in, void* out) {
 
 struct TiledEntry {
   int b, tx, ty, vx, vy;
+  bool general;  // handles x / y groups of several dims
   const void* kernel;
   void (*replay)(const Candidate&, const void*, void*);
 };
@@ -310,10 +311,15 @@ struct PackedEntry {
   void (*replay)(const Candidate&, const void*, void*);
 };
 
-#define LRT_TILED(B, TX, TY, VX, VY) \
-  {B, TX, TY, VX, VY, reinterpret_cast<const void*>(&k_tiled<B, TX, TY, VX, VY>), &replay_tiled<B, TX, TY, VX, VY>},
+#define LRT_TILED_G(B, TX, TY, VX, VY, G)                                                    \
+  {B, TX, TY, VX, VY, G, reinterpret_cast<const void*>(&k_tiled<B, TX, TY, VX, VY, G>), \
+   &replay_tiled<B, TX, TY, VX, VY, G>},
+#define LRT_TILED(B, TX, TY, VX, VY) LRT_TILED_G(B, TX, TY, VX, VY, false)
 #define LRT_TILED_V(B, TX, TY, V) \
   LRT_TILED(B, TX, TY, 1, 1) LRT_TILED(B, TX, TY, V, 1) LRT_TILED(B, TX, TY, 1, V) LRT_TILED(B, TX, TY, V, V)
+#define LRT_TILED_GV(B, TX, TY, V)                                                           \
+  LRT_TILED_G(B, TX, TY, 1, 1, true) LRT_TILED_G(B, TX, TY, V, 1, true) LRT_TILED_G(B, TX, TY, 1, V, true) \
+  LRT_TILED_G(B, TX, TY, V, V, true)
 
 #define LRT_TILED_2(TX, TY) \
   LRT_TILED(4, TX, TY, 2, 1) LRT_TILED(4, TX, TY, 1, 2) LRT_TILED(4, TX, TY, 2, 2) LRT_TILED(4, TX, TY, 4, 2) \
@@ -330,6 +336,12 @@ static const TiledEntry kTiledTable[] = {
     // multiple of 4 (half the wave instructions of 4-byte lanes).
     LRT_TILED_2(64, 64) LRT_TILED_2(32, 32) LRT_TILED_2(64, 32) LRT_TILED_2(32, 64) LRT_TILED_2(128, 64)
     LRT_TILED_2(64, 128)
+    // Planes whose x or y is a group of several dims (separate
+    // instantiations, so that the one-dim kernels above keep their
+    // registers and occupancy).
+    LRT_TILED_GV(4, 64, 64, 4) LRT_TILED_GV(4, 32, 32, 4) LRT_TILED_GV(4, 128, 64, 4) LRT_TILED_GV(4, 64, 128, 4)
+    LRT_TILED_GV(8, 64, 64, 2) LRT_TILED_GV(8, 32, 32, 2) LRT_TILED_GV(8, 128, 64, 2) LRT_TILED_GV(8, 64, 128, 2)
+    LRT_TILED_G(16, 16, 16, 1, 1, true) LRT_TILED_G(16, 32, 32, 1, 1, true)
 };
 
 #define LRT_COPY(W, TX) \
@@ -350,8 +362,10 @@ static const PackedEntry kPackedTable[] = {
 };
 
 static const TiledEntry* find_tiled(const Candidate& c) {
+  const bool general = c.xgroup > 1 || c.ygroup > 1;
   for (const auto& e : kTiledTable)
-    if (e.b == c.elem && e.tx == c.tile_x && e.ty == c.tile_y && e.vx == c.vec_x && e.vy == c.vec_y)
+    if (e.b == c.elem && e.tx == c.tile_x && e.ty == c.tile_y && e.vx == c.vec_x && e.vy == c.vec_y &&
+        e.general == general)
       return &e;
   return nullptr;
 }
@@ -366,9 +380,9 @@ static const PackedEntry* find_packed(const Candidate& c) {
   return nullptr;
 }
 
-bool tiled_shape_available(int elem, int tx, int ty, int vx, int vy) {
+bool tiled_shape_available(int elem, int tx, int ty, int vx, int vy, bool general) {
   for (const auto& e : kTiledTable)
-    if (e.b == elem && e.tx == tx && e.ty == ty && e.vx == vx && e.vy == vy) return true;
+    if (e.b == elem && e.tx == tx && e.ty == ty && e.vx == vx && e.vy == vy && e.general == general) return true;
   return false;
 }
 bool copy_shape_available(int word, int tx) {

@@ -325,7 +325,11 @@ std::string Candidate::describe() const {
   const char* n = (method >= 0 && method < kNumMethods) ? names[method] : "?";
   switch (method) {
     case kTiled:
-      snprintf(buf, sizeof(buf), "%s tile %dx%d vec %d/%d grid %d", n, tile_x, tile_y, vec_x, vec_y, grid_x);
+      if (xgroup > 1 || ygroup > 1)
+        snprintf(buf, sizeof(buf), "%s tile %dx%d vec %d/%d groups %d/%d grid %d", n, tile_x, tile_y, vec_x, vec_y,
+                 xgroup, ygroup, grid_x);
+      else
+        snprintf(buf, sizeof(buf), "%s tile %dx%d vec %d/%d grid %d", n, tile_x, tile_y, vec_x, vec_y, grid_x);
       break;
     case kTiledCopy:
       snprintf(buf, sizeof(buf), "%s word %dB tx %d grid %d", n, word, copy_tx, grid_x);
@@ -360,13 +364,35 @@ namespace {
 // Tiled / TiledCopy candidates (reduced problem)
 // ---------------------------------------------------------------------------
 
-void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& out) {
+// Group map of dims `dims[0..n)` (fastest first) moving the other tensor by
+// `strides`.
+void make_group(const Problem& r, const int* dims, int n, const long long* strides, GroupMap& g) {
+  g.n = n;
+  for (int k = 0; k < kMaxGroup; k++) g.stride[k] = k < n ? (int)strides[dims[k]] : 0;
+  for (int k = 0; k < kMaxGroup - 1; k++) g.div[k] = make_fastdiv(k < n ? (uint32_t)r.dim[dims[k]] : 1u);
+}
+
+// Tiled candidates of the reduced problem whose plane is x = input dims
+// [0, a) and y = output dims [0, b) (the reference's Tiled is a = b = 1).
+void add_tiled_plane(const Problem& r, const DeviceInfo& dev, int a_dims, int b_dims, std::vector<Candidate>& out) {
   long long cin[kMaxRank], cout[kMaxRank];
   input_strides(r, cin);
   output_strides(r, cout);
   const int B = r.elem;
-  const int ydim = r.perm[0];
-  const int X = r.dim[0], Y = r.dim[ydim];
+  int xdims[kMaxGroup], ydims[kMaxGroup];
+  long long X = 1, Y = 1;
+  bool skip[kMaxRank] = {false};
+  for (int k = 0; k < a_dims; k++) {
+    xdims[k] = k;
+    X *= r.dim[k];
+    skip[k] = true;
+  }
+  for (int k = 0; k < b_dims; k++) {
+    ydims[k] = r.perm[k];
+    if (skip[ydims[k]]) return;  // the groups must not share a dim
+    Y *= r.dim[ydims[k]];
+  }
+  for (int k = 0; k < b_dims; k++) skip[ydims[k]] = true;
   const int vmax = 16 / B;  // lanes move at most 16 bytes
   // Tile shapes (x, y). 4- and 8-byte elements also get tiles whose rows
   // are 512 bytes long in one or both directions: on MI355X they move 3-18%
@@ -377,12 +403,13 @@ void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& 
   static const int shapes16[4][2] = {{16, 16}, {32, 32}, {32, 16}, {16, 32}};
   const int (*shapes)[2] = (B == 16) ? shapes16 : (B == 8) ? shapes8 : shapes4;
   const int nshapes = (B == 16) ? 4 : (B == 8) ? 6 : 7;
-  bool skip[kMaxRank] = {false};
-  skip[0] = true;
-  skip[ydim] = true;
   OuterMap outer;
   int vol_outer;
   build_outer(r, skip, cin, cout, 1, 1, outer, vol_outer);
+  GroupMap xg, yg;
+  make_group(r, xdims, a_dims, cout, xg);
+  make_group(r, ydims, b_dims, cin, yg);
+  const bool general = a_dims > 1 || b_dims > 1;
   for (int s = 0; s < nshapes; s++) {
     Candidate c;
     c.method = kTiled;
@@ -390,15 +417,19 @@ void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& 
     c.source = "reduced";
     c.tile_x = shapes[s][0];
     c.tile_y = shapes[s][1];
+    c.xgroup = a_dims;
+    c.ygroup = b_dims;
     // Widest lanes (vx elements per input access, vy per output access)
     // that divide the plane's extents and have a kernel instantiated;
-    // ties go to the larger sum.
+    // ties go to the larger sum. Every input offset outside the x group is
+    // a multiple of X and every output offset outside the y group a
+    // multiple of Y, so X % vx == 0 and Y % vy == 0 keep lanes aligned.
     int vx = 0, vy = 0;
     for (int ax : {4, 2, 1})
       for (int ay : {4, 2, 1}) {
         if (ax > vmax || ay > vmax || X % ax || Y % ay) continue;
-        if (dev.batch_mode ? !batch_tiled_available(B, c.tile_x, c.tile_y, ax, ay)
-                           : !tiled_shape_available(B, c.tile_x, c.tile_y, ax, ay))
+        if (dev.batch_mode ? (general || !batch_tiled_available(B, c.tile_x, c.tile_y, ax, ay))
+                           : !tiled_shape_available(B, c.tile_x, c.tile_y, ax, ay, general))
           continue;
         if (ax * ay > vx * vy || (ax * ay == vx * vy && ax + ay > vx + vy)) {
           vx = ax;
@@ -408,14 +439,15 @@ void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& 
     if (vx == 0) continue;
     c.vec_x = vx;
     c.vec_y = vy;
-    // Skip tiles more than twice the plane in a direction when a smaller one exists.
     TiledArgs& a = c.tiled;
-    a.X = X;
-    a.Y = Y;
-    a.in_ys = (int)cin[ydim];
+    a.X = (int)X;
+    a.Y = (int)Y;
+    a.in_ys = (int)cin[ydims[0]];
     a.out_xs = (int)cout[0];
-    a.ntx = (X + c.tile_x - 1) / c.tile_x;
-    const int nty = (Y + c.tile_y - 1) / c.tile_y;
+    a.xg = xg;
+    a.yg = yg;
+    a.ntx = (int)((X + c.tile_x - 1) / c.tile_x);
+    const int nty = (int)((Y + c.tile_y - 1) / c.tile_y);
     a.ntiles = a.ntx * nty;
     a.ntx_div = make_fastdiv((uint32_t)a.ntx);
     a.ntiles_div = make_fastdiv((uint32_t)a.ntiles);
@@ -426,13 +458,44 @@ void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& 
     c.lds_bytes = c.tile_y * (c.tile_x + 1) * B;
     c.grid_x = std::max(1, std::min(a.items, dev.cu_count * grid_per_cu()));
     c.bytes = 2 * r.vol * B;
-    c.split.sizeMm = 1;
-    c.split.sizeMk = 1;
-    c.split.volMm = X;
-    c.split.volMk = Y;
+    c.split.sizeMm = a_dims;
+    c.split.sizeMk = b_dims;
+    c.split.volMm = (int)X;
+    c.split.volMk = (int)Y;
     c.split.volMbar = vol_outer;
-    c.split.volMmk = X * Y;
+    c.split.volMmk = (int)(X * Y);
     out.push_back(c);
+  }
+}
+
+// Tiled candidates: the plane of input dim 0 and output dim 0 (as the
+// reference), and, where those dims are short, planes whose x is the
+// shortest run of leading input dims reaching 64 elements and whose y is
+// the same on the output side, so that both the reads and the writes of a
+// tile stay long.
+void add_tiled(const Problem& r, const DeviceInfo& dev, std::vector<Candidate>& out) {
+  add_tiled_plane(r, dev, 1, 1, out);
+  auto reach = [&](bool input_side) {
+    long long v = 1;
+    int k = 0;
+    while (k < r.rank && k < kMaxGroup) {
+      v *= r.dim[input_side ? k : r.perm[k]];
+      k++;
+      if (v >= 64) break;
+    }
+    return k;
+  };
+  const int a64 = reach(true), b64 = reach(false);
+  const int combos[3][2] = {{a64, 1}, {1, b64}, {a64, b64}};
+  for (const auto& ab : combos) {
+    if (ab[0] == 1 && ab[1] == 1) continue;
+    if ((ab[0] > 1 && r.dim[0] >= 64) || (ab[1] > 1 && r.dim[r.perm[0]] >= 64)) continue;
+    bool dup = false;
+    for (const auto& q : combos) {
+      if (&q == &ab) break;
+      dup |= q[0] == ab[0] && q[1] == ab[1];
+    }
+    if (!dup) add_tiled_plane(r, dev, ab[0], ab[1], out);
   }
 }
 
@@ -856,8 +919,10 @@ void model_tiled(Candidate& c) {
     int tx = std::min(c.tile_x, a.X - x0), ty = std::min(c.tile_y, a.Y - y0);
     rin.clear();
     rout.clear();
-    for (int y = 0; y < ty; y++) rin.push_back({(oi + x0 + (long long)(y0 + y) * a.in_ys) * B, (long long)tx * B});
-    for (int x = 0; x < tx; x++) rout.push_back({(oo + y0 + (long long)(x0 + x) * a.out_xs) * B, (long long)ty * B});
+    for (int y = 0; y < ty; y++)
+      rin.push_back({(oi + x0 + (long long)tiled_in_y(a, (uint32_t)(y0 + y))) * B, (long long)tx * B});
+    for (int x = 0; x < tx; x++)
+      rout.push_back({(oo + y0 + (long long)tiled_out_x(a, (uint32_t)(x0 + x))) * B, (long long)ty * B});
     for (const Run& r : rin) TR += tran(r.s, r.n);
     for (const Run& r : rout) TW += tran(r.s, r.n);
     double l, p;

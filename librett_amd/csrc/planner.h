@@ -89,8 +89,9 @@ struct Candidate {
   Method method = kUnknown;
   int elem = 0;            // element bytes
   const char* source = ""; // "reduced" or "full"
-  // Tiled
+  // Tiled: tile, lane widths, and the dims in the plane's x and y groups
   int tile_x = 0, tile_y = 0, vec_x = 1, vec_y = 1;
+  int xgroup = 1, ygroup = 1;
   // TiledCopy
   int word = 0, copy_tx = 0;
   // Packed / PackedSplit
