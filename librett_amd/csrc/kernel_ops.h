@@ -141,19 +141,24 @@ struct TiledOps {
 
   __host__ __device__ static void load(const TiledArgs& a, const TileCtx& c, int t,
                                        const E* __restrict__ in, E* lds) {
+    VI r[NPR];
+    fetch(a, c, t, in, r);
+    stash(a, c, t, r, lds);
+  }
+
+  // The two halves of load(): the tile's input rows into registers, then
+  // registers into LDS (the pipelined kernel fetches the next tile between
+  // them and the current tile's store phase).
+  __host__ __device__ static void fetch(const TiledArgs& a, const TileCtx& c, int t, const E* __restrict__ in,
+                                        VI* r) {
     const int lx = (t % LR) * VX;
     const int ly = t / LR;
     const int x = c.x0 + lx;
-    VI r[NPR];
     if (c.full && (!G || a.yg.n == 1)) {
       const E* p = in + (c.off_in + x + (c.y0 + ly) * a.in_ys);
       const int step = RP * a.in_ys;
 #pragma unroll
       for (int j = 0; j < NPR; j++) r[j] = ld(reinterpret_cast<const VI*>(p + j * step));
-#pragma unroll
-      for (int j = 0; j < NPR; j++)
-#pragma unroll
-        for (int e = 0; e < VX; e++) lds[(ly + j * RP) * PITCH + lx + e] = vget<E>(r[j], e);
     } else {
       const bool xin = x < a.X;
 #pragma unroll
@@ -162,13 +167,17 @@ struct TiledOps {
         const int yoff = G ? tiled_in_y(a, (uint32_t)y) : y * a.in_ys;
         if (xin && y < a.Y) r[j] = ld(reinterpret_cast<const VI*>(in + (c.off_in + x + yoff)));
       }
+    }
+  }
+  __host__ __device__ static void stash(const TiledArgs& a, const TileCtx& c, int t, const VI* r, E* lds) {
+    const int lx = (t % LR) * VX;
+    const int ly = t / LR;
+    const bool xin = c.x0 + lx < a.X;
 #pragma unroll
-      for (int j = 0; j < NPR; j++) {
-        const int y = c.y0 + ly + j * RP;
-        if (xin && y < a.Y) {
+    for (int j = 0; j < NPR; j++) {
+      if (c.full || (xin && c.y0 + ly + j * RP < a.Y)) {
 #pragma unroll
-          for (int e = 0; e < VX; e++) lds[(ly + j * RP) * PITCH + lx + e] = vget<E>(r[j], e);
-        }
+        for (int e = 0; e < VX; e++) lds[(ly + j * RP) * PITCH + lx + e] = vget<E>(r[j], e);
       }
     }
   }
