@@ -189,7 +189,10 @@ void selector_features(const Problem& full, const Problem& red, const Candidate&
   const double t_roof = std::max(1.0, (double)c.bytes) / 8.0e12;
   f[46] = lg(m.predicted_s / t_roof);
   f[47] = lg(m.t_stream / t_roof);
-  f[48] = (float)((c.method == kPacked || c.method == kPackedSplit) ? c.packed.order : 0);
+  // Packed work order; for Tiled, the extra dims in the plane's x and y groups.
+  f[48] = (float)((c.method == kPacked || c.method == kPackedSplit) ? c.packed.order
+                  : c.method == kTiled                            ? (c.xgroup - 1) + (c.ygroup - 1)
+                                                                  : 0);
   f[49] = std::strcmp(c.source, "full") == 0 ? 1.0f : 0.0f;
 }
 
