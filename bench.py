@@ -103,6 +103,7 @@ def main():
 
     hip.check(hip.runtime().hipSetDevice(0), "hipSetDevice")
     stream = hip.stream_create()
+    L.librettInitialize()  # one-time set-up, as a LibreTT caller would do first
     if args.analytic:
         L.selector_set(None, None, None)
     sub = {"batched": bench_batched, "serving": bench_serving, "selector": bench_selector,

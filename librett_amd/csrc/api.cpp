@@ -152,45 +152,6 @@ std::string cache_key(const Problem& p, int device, bool measured) {
   return k;
 }
 
-int needed_alignment(const Candidate& c) {
-  switch (c.method) {
-    case kTiled:
-      return c.elem * std::max(c.vec_x, c.vec_y);
-    case kTiledCopy:
-      return c.word;
-    default:
-      return c.elem;
-  }
-}
-
-Candidate copy_fallback(const Candidate& c) {
-  // TiledCopy moves W-byte words; for pointers aligned only to the element,
-  // rebuild the arguments with W = element size.
-  Candidate f = c;
-  if (c.method != kTiledCopy || c.word == c.elem) return f;
-  const int k = c.word / c.elem;
-  CopyArgs& a = f.copy;
-  a.X *= k;
-  a.in_ys *= k;
-  a.out_ys *= k;
-  for (int r = 0; r < a.outer.n; r++) {
-    a.outer.ct_in[r] *= k;
-    a.outer.ct_out[r] *= k;
-  }
-  f.word = c.elem;
-  f.copy_tx = 64;
-  const int ty = (256 / f.copy_tx) * 4;
-  a.ntx = (a.X + f.copy_tx - 1) / f.copy_tx;
-  const int nty = (a.Y + ty - 1) / ty;
-  const long long vol_outer = c.copy.items / std::max(1, c.copy.ntiles);
-  a.ntiles = a.ntx * nty;
-  a.ntx_div = make_fastdiv((uint32_t)a.ntx);
-  a.ntiles_div = make_fastdiv((uint32_t)a.ntiles);
-  a.items = (int)(a.ntiles * vol_outer);
-  f.grid_x = std::max(1, std::min(a.items, 16384));
-  return f;
-}
-
 std::shared_ptr<PlanObj> make_plan_obj(const Problem& full, const Candidate& c, int device, hipStream_t stream) {
   auto p = std::make_shared<PlanObj>();
   p->device = device;
@@ -198,12 +159,7 @@ std::shared_ptr<PlanObj> make_plan_obj(const Problem& full, const Candidate& c, 
   p->full = full;
   p->cand = c;
   p->align = needed_alignment(c);
-  if (c.method == kTiled)
-    p->fallback = scalar_fallback(c);
-  else if (c.method == kTiledCopy)
-    p->fallback = copy_fallback(c);
-  else
-    p->fallback = c;
+  p->fallback = unaligned_fallback(c);
   return p;
 }
 
@@ -413,7 +369,17 @@ using namespace lrt;
 // librett.h (C++ linkage, same mangled names as the reference)
 // ---------------------------------------------------------------------------
 
-void librettInitialize() {}
+// Optional (every entry point sets up what it needs on first use): takes the
+// one-time set-up -- device properties, the planning threads, the learned
+// selector's weights and inference buffers -- out of the first plan. The
+// reference chooses its memory allocator here (src/librett.cpp:387-395);
+// this library's allocator hook is librett_c_set_allocator.
+void librettInitialize() {
+  DeviceInfo dev;
+  if (!device_info(dev)) return;
+  planner_warm_up();
+  selector_prepare(dev.device_id);
+}
 
 void librettFinalize() {}
 

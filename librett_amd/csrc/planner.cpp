@@ -165,6 +165,12 @@ class PlanPool {
   uint64_t gen_ = 0;
 };
 
+}  // namespace
+
+void planner_warm_up() { (void)PlanPool::get(); }
+
+namespace {
+
 // f(0..n-1), on the planning threads when there is enough work.
 void parallel_for(int n, const std::function<void(int)>& f) {
   if (n >= 8) {
@@ -1342,6 +1348,51 @@ Candidate scalar_fallback(const Candidate& c) {
     f.vec_y = 1;
   }
   return f;
+}
+
+int needed_alignment(const Candidate& c) {
+  switch (c.method) {
+    case kTiled:
+      return c.elem * std::max(c.vec_x, c.vec_y);
+    case kTiledCopy:
+      return c.word;
+    default:
+      return c.elem;
+  }
+}
+
+Candidate copy_fallback(const Candidate& c) {
+  // TiledCopy moves W-byte words; for pointers aligned only to the element,
+  // rebuild the arguments with W = element size.
+  Candidate f = c;
+  if (c.method != kTiledCopy || c.word == c.elem) return f;
+  const int k = c.word / c.elem;
+  CopyArgs& a = f.copy;
+  a.X *= k;
+  a.in_ys *= k;
+  a.out_ys *= k;
+  for (int r = 0; r < a.outer.n; r++) {
+    a.outer.ct_in[r] *= k;
+    a.outer.ct_out[r] *= k;
+  }
+  f.word = c.elem;
+  f.copy_tx = 64;
+  const int ty = (256 / f.copy_tx) * 4;
+  a.ntx = (a.X + f.copy_tx - 1) / f.copy_tx;
+  const int nty = (a.Y + ty - 1) / ty;
+  const long long vol_outer = c.copy.items / std::max(1, c.copy.ntiles);
+  a.ntiles = a.ntx * nty;
+  a.ntx_div = make_fastdiv((uint32_t)a.ntx);
+  a.ntiles_div = make_fastdiv((uint32_t)a.ntiles);
+  a.items = (int)(a.ntiles * vol_outer);
+  f.grid_x = std::max(1, std::min(a.items, 16384));
+  return f;
+}
+
+Candidate unaligned_fallback(const Candidate& c) {
+  if (c.method == kTiled) return scalar_fallback(c);
+  if (c.method == kTiledCopy) return copy_fallback(c);
+  return c;
 }
 
 }  // namespace lrt

@@ -140,9 +140,23 @@ void set_tile_group(TiledArgs& a, int nty, int group);
 // caller's pointers are not aligned to the vector width.
 Candidate scalar_fallback(const Candidate& c);
 
+// Pointer alignment a candidate's vector accesses need.
+int needed_alignment(const Candidate& c);
+
+// Variant of a TiledCopy candidate that moves element-sized words.
+Candidate copy_fallback(const Candidate& c);
+
+// The candidate to launch when the pointers are not aligned to
+// needed_alignment(c): scalar Tiled lanes, element-sized TiledCopy words,
+// or c itself (Packed and Trivial need only element alignment).
+Candidate unaligned_fallback(const Candidate& c);
+
 // Contiguous runs (start offset, length in elements) of one staged shape
 // walked in LDS order (input side) or output order (output side).
 void staged_runs(const StagedShape& s, bool output_side, std::vector<std::pair<long long, int>>& runs);
+
+// Starts the planning threads now rather than in the first plan.
+void planner_warm_up();
 
 // Active blocks per CU for a candidate's kernel (runtime occupancy query if
 // a GPU is present, otherwise an LDS / wave-slot estimate).

@@ -283,6 +283,37 @@ int selector_choose(const std::vector<Candidate>& cands, const Problem& full, in
   return best;
 }
 
+void selector_prepare(int device) {
+  load_from_env();
+  Selector& s = state();
+  std::lock_guard<std::mutex> lock(s.mu);
+  if (!s.gen) return;
+  DevBuf& d = s.dev[device];
+  bool ok = true;
+  if (!d.stream) ok = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) == hipSuccess;
+  if (ok && !d.params) {
+    ok = dev_malloc((void**)&d.params, (size_t)kSelMaxModels * mlp::P * sizeof(float)) == hipSuccess;
+    if (!ok) d.params = nullptr;
+    d.gen = 0;
+  }
+  if (ok && d.cap == 0) {
+    dev_free(d.x);
+    dev_free(d.out);
+    d.x = d.out = nullptr;
+    const int cap = 256;
+    ok = dev_malloc((void**)&d.x, (size_t)cap * mlp::F * sizeof(float)) == hipSuccess &&
+         dev_malloc((void**)&d.out, (size_t)kSelMaxModels * cap * sizeof(float)) == hipSuccess;
+    if (ok) d.cap = cap;
+  }
+  if (ok && d.gen != s.gen) {
+    ok = hipMemcpyAsync(d.params, s.params.data(), s.params.size() * sizeof(float), hipMemcpyHostToDevice,
+                        d.stream) == hipSuccess &&
+         hipStreamSynchronize(d.stream) == hipSuccess;
+    if (ok) d.gen = s.gen;
+  }
+  if (!ok) (void)hipGetLastError();
+}
+
 void selector_release_buffers(const std::function<void()>& then) {
   Selector& s = state();
   std::lock_guard<std::mutex> lock(s.mu);

@@ -51,6 +51,10 @@ void selector_features(const Problem& full, const Problem& reduced, const Candid
 // kernel on the current device.
 int selector_choose(const std::vector<Candidate>& cands, const Problem& full, int device);
 
+// Loads the weights (if any) and makes the device's inference buffers now
+// rather than in the first plan.
+void selector_prepare(int device);
+
 // Returns the per-device inference buffers (taken from the device allocator,
 // kept between plans) to the allocator that made them, then runs `then`
 // with the selector locked, so no plan can take a buffer in between. The

@@ -122,3 +122,25 @@ def test_errors():
         L.host_candidate_counters((8, 8), (1, 0), 4, 999)
     with pytest.raises(RuntimeError):
         L.host_access_trace((8, 8), (1, 0), 4, 0, 2)
+
+
+def test_by_definition_counts_hand_worked():
+    # Accesses (item, store, instruction, thread, byte offset, bytes) whose
+    # counts are worked out by hand, including lines written only in part
+    # and a line completed by two instructions of one item.
+    rec = []
+    for t in range(64):  # item 0: one load, bytes 0..255 -> 2 segments, 2 lines
+        rec.append((0, 0, 0, t, 4 * t, 4))
+    for t in range(64):  # item 0: one store, bytes 64..319 -> lines 0 (64 B), 1 (full), 2 (64 B)
+        rec.append((0, 1, 0, t, 64 + 4 * t, 4))
+    for t in range(16):  # item 1, wave 0: 16-byte lanes, bytes 1000..1255 -> lines 7 (24 B), 8, 9 (104 B)
+        rec.append((1, 1, 0, t, 1000 + 16 * t, 16))
+    for t in range(64, 128):  # item 1, wave 1: bytes 2048..2303 -> lines 16, 17, both full
+        rec.append((1, 1, 0, t, 2048 + 4 * (t - 64), 4))
+    for t in range(16):  # item 2: two instructions fill line 32 together
+        rec.append((2, 1, 0, t, 4096 + 4 * t, 4))
+        rec.append((2, 1, 1, t, 4160 + 4 * t, 4))
+    got = OC.counts(np.array(rec))
+    assert got == {"ld_req": 1, "ld_tran": 2, "ld_bytes": 256, "read_lines": 2,
+                   "st_req": 5, "st_tran": 3 + 3 + 2 + 1 + 1, "st_bytes": 256 + 256 + 256 + 128,
+                   "write_lines": 3 + 3 + 2 + 1, "partial_lines": 2 + 2 + 0 + 0, "items": 3}
