@@ -24,7 +24,6 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
-#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -39,10 +38,8 @@
 #include "devmem.h"
 #include "../../include/librett_c.h"
 #include "batch.h"
-#include "kernels.h"
 #include "lru_cache.h"
 #include "planner.h"
-#include "selector.h"
 
 namespace lrt {
 namespace {
@@ -139,11 +136,6 @@ struct Request {
   unsigned long long id = 0;
   Problem prob;
   Candidate plan;
-  // Large requests run their own kernel with the single-problem plan
-  // (librettPlan's choice) instead of a share of the persistent kernel.
-  bool direct = false;
-  Candidate fallback;        // direct: the plan for pointers not aligned to `align`
-  int align = 1;
   const void* in = nullptr;
   void* out = nullptr;
   bool release_out = false;  // generator requests return their output to the arena
@@ -230,37 +222,15 @@ class Server {
 
   Arena& arena() { return arena_; }
 
-  // Requests of at least this many bytes per tensor run as their own
-  // kernel (LIBRETT_SERVE_DIRECT_MB; 0 keeps every request in the batch).
-  static size_t direct_bytes() {
-    static const size_t v = [] {
-      const char* e = std::getenv("LIBRETT_SERVE_DIRECT_MB");
-      const double mb = (e && *e) ? std::atof(e) : 8.0;
-      return mb > 0 ? (size_t)(mb * (1 << 20)) : (size_t)-1;
-    }();
-    return v;
-  }
-
   // Plan of a problem from the LRU cache, planned on a miss (in the calling
-  // thread; the time misses take is part of the statistics). Small requests
-  // get the batch-mode plan; large ones the plan librettPlan would choose.
-  bool plan_for(const Problem& p, Candidate& c, bool direct = false) {
-    const std::string key = (direct ? "D" : "B") + key_of(p);
+  // thread; the time misses take is part of the statistics).
+  bool plan_for(const Problem& p, Candidate& c) {
+    const std::string key = key_of(p);
     if (plans_.get(key, c)) return true;
     const double t0 = now_s();
-    if (direct) {
-      std::vector<Candidate> cands = make_candidates(p, dev_);
-      if (cands.empty()) return false;
-      int best = selector_choose(cands, p, device_);
-      if (best < 0) best = choose_by_model(cands);
-      if (best < 0) return false;
-      c = cands[best];
-      if (prepare_kernel(c) != 0 || prepare_kernel(unaligned_fallback(c)) != 0) return false;
-    } else {
-      std::vector<Candidate> cands = batch_candidates(p, dev_);
-      if (cands.empty()) return false;
-      c = cands[0];
-    }
+    std::vector<Candidate> cands = batch_candidates(p, dev_);
+    if (cands.empty()) return false;
+    c = cands[0];
     plans_.put(key, c);
     const double dt = now_s() - t0;
     std::lock_guard<std::mutex> lock(mu_);
@@ -273,12 +243,7 @@ class Server {
              unsigned long long* ticket, bool release_in = false) {
     auto r = std::make_shared<Request>();
     r->prob = p;
-    r->direct = (size_t)p.vol * p.elem >= direct_bytes();
-    if (!plan_for(p, r->plan, r->direct)) return LIBRETT_C_INTERNAL_ERROR;
-    if (r->direct) {
-      r->fallback = unaligned_fallback(r->plan);
-      r->align = needed_alignment(r->plan);
-    }
+    if (!plan_for(p, r->plan)) return LIBRETT_C_INTERNAL_ERROR;
     r->in = in;
     r->out = out;
     r->release_out = release_out;
@@ -394,16 +359,8 @@ class Server {
     std::vector<BatchDesc> ds, fs;
     std::vector<long long> offs;
     std::vector<BatchDesc> use;
-    std::vector<std::shared_ptr<Request>> direct;
     while (!pending_.empty() && (int)take.size() < max_batch_) {
       std::shared_ptr<Request> r = pending_.front();
-      if (r->direct) {
-        pending_.pop_front();
-        r->status = 1;
-        take.push_back(r);
-        direct.push_back(r);
-        continue;
-      }
       if (!describe_problem(r->prob, r->plan, ds, fs, offs)) {
         pending_.pop_front();
         r->status = -1;
@@ -429,33 +386,24 @@ class Server {
       take.push_back(r);
     }
     if (take.empty()) return false;
-    bool ok = true;
-    // Large requests first, each its own kernel on the slot's stream.
-    for (auto& r : direct) {
-      const bool mis = (((uintptr_t)r->in | (uintptr_t)r->out) & (uintptr_t)(r->align - 1)) != 0;
-      ok = ok && launch_candidate(mis ? r->fallback : r->plan, r->in, r->out, s.stream) == hipSuccess;
-    }
-    if (ndesc > 0) {
-      long long bytes = 0;
-      for (auto& r : take)
-        if (!r->direct) bytes += 2LL * r->prob.vol * r->prob.elem;
-      const long long unit_bytes = batch_unit_bytes(bytes, grid_max_);
-      for (int k = 0; k < ndesc; k++) batch_regroup(use[k], unit_bytes);
-      // One launch per kernel family, in family order on the slot's stream.
-      BatchLayout L;
-      batch_layout(use, L, nullptr);
-      bool fits = true;
-      for (int f = 0; f < kBatchFamilies; f++) fits = fits && L.total[f] < (1LL << 31);
-      for (int k = 0; k < ndesc; k++) s.h_desc[k] = use[k];
-      for (size_t k = 0; k < L.start.size(); k++) s.h_start[k] = L.start[k];
-      ok = ok && fits &&
-           hipMemcpyAsync(s.d_desc, s.h_desc, ndesc * sizeof(BatchDesc), hipMemcpyHostToDevice, s.stream) ==
-               hipSuccess &&
-           hipMemcpyAsync(s.d_start, s.h_start, L.start.size() * sizeof(int), hipMemcpyHostToDevice, s.stream) ==
-               hipSuccess &&
-           launch_batch_all(s.d_desc, s.d_start, L, grid_, s.stream) == hipSuccess;
-    }
-    ok = ok && hipEventRecord(s.done, s.stream) == hipSuccess;
+    long long bytes = 0;
+    for (auto& r : take) bytes += 2LL * r->prob.vol * r->prob.elem;
+    const long long unit_bytes = batch_unit_bytes(bytes, grid_max_);
+    for (int k = 0; k < ndesc; k++) batch_regroup(use[k], unit_bytes);
+    // One launch per kernel family, in family order on the slot's stream.
+    BatchLayout L;
+    batch_layout(use, L, nullptr);
+    bool fits = true;
+    for (int f = 0; f < kBatchFamilies; f++) fits = fits && L.total[f] < (1LL << 31);
+    for (int k = 0; k < ndesc; k++) s.h_desc[k] = use[k];
+    for (size_t k = 0; k < L.start.size(); k++) s.h_start[k] = L.start[k];
+    bool ok = fits &&
+              hipMemcpyAsync(s.d_desc, s.h_desc, ndesc * sizeof(BatchDesc), hipMemcpyHostToDevice, s.stream) ==
+                  hipSuccess &&
+              hipMemcpyAsync(s.d_start, s.h_start, L.start.size() * sizeof(int), hipMemcpyHostToDevice, s.stream) ==
+                  hipSuccess &&
+              launch_batch_all(s.d_desc, s.d_start, L, grid_, s.stream) == hipSuccess &&
+              hipEventRecord(s.done, s.stream) == hipSuccess;
     if (!ok) {
       (void)hipGetLastError();
       for (auto& r : take) {
