@@ -348,7 +348,7 @@ int librett_c_selector_load(const char* path);
 unsigned long long librett_c_selector_generation(void);
 
 /* librettPlan keeps the analytic model's choice unless the selector scores
- * its own pick lower by more than this margin (log2 of predicted time; 0.1
+ * its own pick lower by more than this margin (log2 of predicted time; 0.03
  * by default, LIBRETT_SELECTOR_MARGIN overrides). */
 double librett_c_selector_margin(void);
 

@@ -67,10 +67,10 @@ uint64_t selector_generation();
 
 // The selector overrides the analytic model's choice only when its scores
 // (predicted log2 time over the roofline time) put its own pick below the
-// analytic choice by more than this margin. 0.1 (about 7%) by default;
+// analytic choice by more than this margin. 0.03 (about 2%) by default;
 // LIBRETT_SELECTOR_MARGIN sets another value (0: the selector's pick
 // always).
-constexpr double kSelMargin = 0.1;
+constexpr double kSelMargin = 0.03;
 double selector_margin();
 
 }  // namespace lrt

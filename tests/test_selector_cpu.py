@@ -248,7 +248,7 @@ def test_margin_default_and_override():
     env = dict(os.environ)
     env.pop("LIBRETT_SELECTOR_MARGIN", None)
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, check=True)
-    assert float(out.stdout) == 0.1
+    assert float(out.stdout) == 0.03
     env["LIBRETT_SELECTOR_MARGIN"] = "0"
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, check=True)
     assert float(out.stdout) == 0.0
