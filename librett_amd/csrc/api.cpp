@@ -378,6 +378,7 @@ void librettInitialize() {
   DeviceInfo dev;
   if (!device_info(dev)) return;
   planner_warm_up();
+  kernels_warm_up();
   selector_prepare(dev.device_id);
 }
 

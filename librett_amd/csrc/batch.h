@@ -56,8 +56,8 @@ long long batch_unit_bytes(long long total_bytes, int grid);
 
 // The persistent kernel comes in one instantiation per family of shapes, so
 // that each holds only the registers its own shapes need.
-enum BatchFamily : int { kFamTiled = 0, kFamCopy = 1, kFamPacked = 2 };
-constexpr int kBatchFamilies = 3;
+enum BatchFamily : int { kFamTiled = 0, kFamCopy = 1, kFamPacked = 2, kFamTiledG = 3 };
+constexpr int kBatchFamilies = 4;
 
 // Family of a shape-table index.
 int batch_family(int variant);
@@ -66,10 +66,10 @@ int batch_family(int variant);
 // [first[f], first[f] + n[f]) and start[start_off[f] ...] holds the first
 // unit of each of them and then the family's unit total.
 struct BatchLayout {
-  int n[kBatchFamilies] = {0, 0, 0};
-  int first[kBatchFamilies] = {0, 0, 0};
-  long long total[kBatchFamilies] = {0, 0, 0};
-  int start_off[kBatchFamilies] = {0, 0, 0};
+  int n[kBatchFamilies] = {};
+  int first[kBatchFamilies] = {};
+  long long total[kBatchFamilies] = {};
+  int start_off[kBatchFamilies] = {};
   std::vector<int> start;  // descriptors + kBatchFamilies entries
 };
 

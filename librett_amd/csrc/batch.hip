@@ -14,30 +14,38 @@ namespace lrt {
 
 namespace {
 
-// Shape table: method, element bytes, then per method
-//   Tiled:     tile_x, tile_y, vec_x, vec_y
+// Shape table: id, method, element bytes, four method parameters, group flag
+//   Tiled:     tile_x, tile_y, vec_x, vec_y; group flag 1 for planes over
+//              dim groups (their own family, so the one-dim kernels keep
+//              their registers)
 //   TiledCopy: word, copy_tx
 //   Packed:    nreg
-#define LRT_BATCH_TABLE(X)                                              \
-  X(0, kTiled, 4, 64, 64, 1, 1) X(1, kTiled, 4, 64, 64, 4, 1)           \
-  X(2, kTiled, 4, 64, 64, 1, 4) X(3, kTiled, 4, 64, 64, 4, 4)           \
-  X(4, kTiled, 8, 32, 32, 1, 1) X(5, kTiled, 8, 32, 32, 2, 1)           \
-  X(6, kTiled, 8, 32, 32, 1, 2) X(7, kTiled, 8, 32, 32, 2, 2)           \
-  X(8, kTiled, 16, 32, 32, 1, 1)                                        \
-  X(9, kTiledCopy, 4, 4, 16, 0, 0) X(10, kTiledCopy, 4, 4, 64, 0, 0)    \
-  X(11, kTiledCopy, 8, 8, 16, 0, 0) X(12, kTiledCopy, 8, 8, 64, 0, 0)   \
-  X(13, kTiledCopy, 16, 16, 16, 0, 0) X(14, kTiledCopy, 16, 16, 64, 0, 0) \
-  X(15, kPacked, 4, 1, 0, 0, 0) X(16, kPacked, 4, 2, 0, 0, 0)           \
-  X(17, kPacked, 4, 4, 0, 0, 0) X(18, kPacked, 4, 8, 0, 0, 0)           \
-  X(19, kPacked, 8, 1, 0, 0, 0) X(20, kPacked, 8, 2, 0, 0, 0)           \
-  X(21, kPacked, 8, 4, 0, 0, 0) X(22, kPacked, 8, 8, 0, 0, 0)           \
-  X(23, kPacked, 16, 1, 0, 0, 0) X(24, kPacked, 16, 2, 0, 0, 0)         \
-  X(25, kPacked, 16, 4, 0, 0, 0)
+#define LRT_BATCH_TABLE(X)                                                  \
+  X(0, kTiled, 4, 64, 64, 1, 1, 0) X(1, kTiled, 4, 64, 64, 4, 1, 0)         \
+  X(2, kTiled, 4, 64, 64, 1, 4, 0) X(3, kTiled, 4, 64, 64, 4, 4, 0)         \
+  X(4, kTiled, 8, 32, 32, 1, 1, 0) X(5, kTiled, 8, 32, 32, 2, 1, 0)         \
+  X(6, kTiled, 8, 32, 32, 1, 2, 0) X(7, kTiled, 8, 32, 32, 2, 2, 0)         \
+  X(8, kTiled, 16, 32, 32, 1, 1, 0)                                         \
+  X(9, kTiledCopy, 4, 4, 16, 0, 0, 0) X(10, kTiledCopy, 4, 4, 64, 0, 0, 0)  \
+  X(11, kTiledCopy, 8, 8, 16, 0, 0, 0) X(12, kTiledCopy, 8, 8, 64, 0, 0, 0) \
+  X(13, kTiledCopy, 16, 16, 16, 0, 0, 0)                                    \
+  X(14, kTiledCopy, 16, 16, 64, 0, 0, 0)                                    \
+  X(15, kPacked, 4, 1, 0, 0, 0, 0) X(16, kPacked, 4, 2, 0, 0, 0, 0)         \
+  X(17, kPacked, 4, 4, 0, 0, 0, 0) X(18, kPacked, 4, 8, 0, 0, 0, 0)         \
+  X(19, kPacked, 8, 1, 0, 0, 0, 0) X(20, kPacked, 8, 2, 0, 0, 0, 0)         \
+  X(21, kPacked, 8, 4, 0, 0, 0, 0) X(22, kPacked, 8, 8, 0, 0, 0, 0)         \
+  X(23, kPacked, 16, 1, 0, 0, 0, 0) X(24, kPacked, 16, 2, 0, 0, 0, 0)       \
+  X(25, kPacked, 16, 4, 0, 0, 0, 0)                                         \
+  X(26, kTiled, 4, 64, 64, 1, 1, 1) X(27, kTiled, 4, 64, 64, 4, 1, 1)       \
+  X(28, kTiled, 4, 64, 64, 1, 4, 1) X(29, kTiled, 4, 64, 64, 4, 4, 1)       \
+  X(30, kTiled, 8, 32, 32, 1, 1, 1) X(31, kTiled, 8, 32, 32, 2, 1, 1)       \
+  X(32, kTiled, 8, 32, 32, 1, 2, 1) X(33, kTiled, 8, 32, 32, 2, 2, 1)       \
+  X(34, kTiled, 16, 32, 32, 1, 1, 1)
 
 struct BatchShape {
-  int id, method, b, p1, p2, p3, p4;
+  int id, method, b, p1, p2, p3, p4, g;
 };
-#define LRT_BATCH_ROW(ID, M, B, P1, P2, P3, P4) {ID, M, B, P1, P2, P3, P4},
+#define LRT_BATCH_ROW(ID, M, B, P1, P2, P3, P4, G) {ID, M, B, P1, P2, P3, P4, G},
 const BatchShape kBatchShapes[] = {LRT_BATCH_TABLE(LRT_BATCH_ROW)};
 #undef LRT_BATCH_ROW
 
@@ -53,9 +61,9 @@ __device__ __forceinline__ int batch_find(const int* __restrict__ start, int npr
   return lo;
 }
 
-template <int B, int TX, int TY, int VX, int VY>
+template <int B, int TX, int TY, int VX, int VY, bool G = false>
 __device__ __forceinline__ void dev_tiled(const BatchDesc& D, int lu, unsigned char* lds_raw) {
-  using Ops = TiledOps<B, TX, TY, VX, VY>;
+  using Ops = TiledOps<B, TX, TY, VX, VY, G>;
   using E = typename Ops::E;
   E* lds = reinterpret_cast<E*>(lds_raw);
   for (int w = lu; w < D.items; w += D.groups) {
@@ -115,6 +123,19 @@ __device__ __forceinline__ void dev_unit(const BatchDesc& D, int lu, unsigned ch
       case 8: dev_tiled<16, 32, 32, 1, 1>(D, lu, lds); break;
       default: break;
     }
+  } else if constexpr (FAM == kFamTiledG) {
+    switch (D.variant) {
+      case 26: dev_tiled<4, 64, 64, 1, 1, true>(D, lu, lds); break;
+      case 27: dev_tiled<4, 64, 64, 4, 1, true>(D, lu, lds); break;
+      case 28: dev_tiled<4, 64, 64, 1, 4, true>(D, lu, lds); break;
+      case 29: dev_tiled<4, 64, 64, 4, 4, true>(D, lu, lds); break;
+      case 30: dev_tiled<8, 32, 32, 1, 1, true>(D, lu, lds); break;
+      case 31: dev_tiled<8, 32, 32, 2, 1, true>(D, lu, lds); break;
+      case 32: dev_tiled<8, 32, 32, 1, 2, true>(D, lu, lds); break;
+      case 33: dev_tiled<8, 32, 32, 2, 2, true>(D, lu, lds); break;
+      case 34: dev_tiled<16, 32, 32, 1, 1, true>(D, lu, lds); break;
+      default: break;
+    }
   } else if constexpr (FAM == kFamCopy) {
     switch (D.variant) {
       case 9: dev_copy<4, 16>(D, lu); break;
@@ -162,6 +183,7 @@ __global__ __launch_bounds__(256) void k_batch(const BatchDesc* __restrict__ des
 const void* batch_kernel(int fam) {
   switch (fam) {
     case kFamTiled: return reinterpret_cast<const void*>(&k_batch<kFamTiled>);
+    case kFamTiledG: return reinterpret_cast<const void*>(&k_batch<kFamTiledG>);
     case kFamCopy: return reinterpret_cast<const void*>(&k_batch<kFamCopy>);
     default: return reinterpret_cast<const void*>(&k_batch<kFamPacked>);
   }
@@ -171,9 +193,9 @@ const void* batch_kernel(int fam) {
 // Host replay: the same per-thread code, workgroup by workgroup.
 // ---------------------------------------------------------------------------
 
-template <int B, int TX, int TY, int VX, int VY>
+template <int B, int TX, int TY, int VX, int VY, bool G = false>
 void host_tiled(const BatchDesc& D, int lu, unsigned char* lds_raw) {
-  using Ops = TiledOps<B, TX, TY, VX, VY>;
+  using Ops = TiledOps<B, TX, TY, VX, VY, G>;
   using E = typename Ops::E;
   E* lds = reinterpret_cast<E*>(lds_raw);
   for (int w = lu; w < D.items; w += D.groups) {
@@ -239,15 +261,26 @@ void host_unit(const BatchDesc& D, int lu, unsigned char* lds) {
     case 23: host_packed<16, 1>(D, lu, lds); break;
     case 24: host_packed<16, 2>(D, lu, lds); break;
     case 25: host_packed<16, 4>(D, lu, lds); break;
+    case 26: host_tiled<4, 64, 64, 1, 1, true>(D, lu, lds); break;
+    case 27: host_tiled<4, 64, 64, 4, 1, true>(D, lu, lds); break;
+    case 28: host_tiled<4, 64, 64, 1, 4, true>(D, lu, lds); break;
+    case 29: host_tiled<4, 64, 64, 4, 4, true>(D, lu, lds); break;
+    case 30: host_tiled<8, 32, 32, 1, 1, true>(D, lu, lds); break;
+    case 31: host_tiled<8, 32, 32, 2, 1, true>(D, lu, lds); break;
+    case 32: host_tiled<8, 32, 32, 1, 2, true>(D, lu, lds); break;
+    case 33: host_tiled<8, 32, 32, 2, 2, true>(D, lu, lds); break;
+    case 34: host_tiled<16, 32, 32, 1, 1, true>(D, lu, lds); break;
     default: break;
   }
 }
 
 }  // namespace
 
-bool batch_tiled_available(int elem, int tx, int ty, int vx, int vy) {
+bool batch_tiled_available(int elem, int tx, int ty, int vx, int vy, bool general) {
   for (const auto& s : kBatchShapes)
-    if (s.method == kTiled && s.b == elem && s.p1 == tx && s.p2 == ty && s.p3 == vx && s.p4 == vy) return true;
+    if (s.method == kTiled && s.b == elem && s.p1 == tx && s.p2 == ty && s.p3 == vx && s.p4 == vy &&
+        (s.g != 0) == general)
+      return true;
   return false;
 }
 
@@ -268,7 +301,7 @@ int batch_variant(const Candidate& c) {
     switch (c.method) {
       case kTiled:
         if (s.method == kTiled && s.b == c.elem && s.p1 == c.tile_x && s.p2 == c.tile_y && s.p3 == c.vec_x &&
-            s.p4 == c.vec_y)
+            s.p4 == c.vec_y && (s.g != 0) == (c.xgroup > 1 || c.ygroup > 1))
           return s.id;
         break;
       case kTiledCopy:
@@ -337,7 +370,9 @@ long long batch_unit_bytes(long long total_bytes, int grid) {
   return std::max<long long>(16 << 10, std::min<long long>(kBatchUnitBytes, per));
 }
 
-int batch_family(int variant) { return variant <= 8 ? kFamTiled : variant <= 14 ? kFamCopy : kFamPacked; }
+int batch_family(int variant) {
+  return variant <= 8 ? kFamTiled : variant <= 14 ? kFamCopy : variant <= 25 ? kFamPacked : kFamTiledG;
+}
 
 hipError_t launch_batch(int fam, const BatchDesc* d_descs, const int* d_start, int nprob, int total, int grid,
                         hipStream_t stream) {

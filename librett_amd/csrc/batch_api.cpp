@@ -102,7 +102,7 @@ struct BatchObj {
   std::vector<long long> byte_offset;   // offset of the descriptor's data in its problem
   BatchLayout layout;                   // descriptors grouped by kernel family
   long long total = 0;                  // units of all families
-  int grid[kBatchFamilies] = {0, 0, 0}; // workgroups of each family's launch
+  int grid[kBatchFamilies] = {}; // workgroups of each family's launch
   int grid_max = 0;
   long long unit_bytes = kBatchUnitBytes;
   BatchDesc* d_descs = nullptr;

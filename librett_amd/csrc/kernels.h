@@ -25,10 +25,15 @@ int prepare_kernel(const Candidate& c);
 // per 256*k words.
 hipError_t launch_device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream, int k, int nt, int grid);
 
+// Loads the transpose kernels' code object on the current device (the
+// runtime does it on first use, which would otherwise land in the first
+// plan's occupancy queries).
+void kernels_warm_up();
+
 // general: a kernel for planes whose x or y is a group of several dims.
 bool tiled_shape_available(int elem, int tx, int ty, int vx, int vy, bool general = false);
 // Shapes the batched engine's persistent kernel is instantiated for (batch.hip).
-bool batch_tiled_available(int elem, int tx, int ty, int vx, int vy);
+bool batch_tiled_available(int elem, int tx, int ty, int vx, int vy, bool general = false);
 bool batch_copy_available(int word, int tx);
 bool batch_packed_available(int elem, int nreg);
 bool copy_shape_available(int word, int tx);

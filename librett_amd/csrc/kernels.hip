@@ -176,6 +176,12 @@ hipError_t launch_device_copy(void* dst, const void* src, size_t bytes, hipStrea
   return hipLaunchKernel(kern, dim3(grid), dim3(256), args, 0, stream);
 }
 
+void kernels_warm_up() {
+  hipFuncAttributes attr;
+  if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&k_devcopy<1, false>)) != hipSuccess)
+    (void)hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Host replays: the same per-thread code, barrier order kept by running every
 // thread's load() before any thread's store().

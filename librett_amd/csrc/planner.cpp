@@ -434,7 +434,7 @@ void add_tiled_plane(const Problem& r, const DeviceInfo& dev, int a_dims, int b_
     for (int ax : {4, 2, 1})
       for (int ay : {4, 2, 1}) {
         if (ax > vmax || ay > vmax || X % ax || Y % ay) continue;
-        if (dev.batch_mode ? (general || !batch_tiled_available(B, c.tile_x, c.tile_y, ax, ay))
+        if (dev.batch_mode ? !batch_tiled_available(B, c.tile_x, c.tile_y, ax, ay, general)
                            : !tiled_shape_available(B, c.tile_x, c.tile_y, ax, ay, general))
           continue;
         if (ax * ay > vx * vy || (ax * ay == vx * vy && ax + ay > vx + vy)) {

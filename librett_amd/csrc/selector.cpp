@@ -41,6 +41,7 @@ struct DevBuf {
   int cap = 0;
   uint64_t gen = 0;
   hipStream_t stream = nullptr;
+  bool warm = false;  // the inference kernel has run once on this device
 };
 
 // Weights and per-device inference buffers. The buffers live until the
@@ -310,6 +311,14 @@ void selector_prepare(int device) {
                         d.stream) == hipSuccess &&
          hipStreamSynchronize(d.stream) == hipSuccess;
     if (ok) d.gen = s.gen;
+  }
+  // One inference on zeros, so that loading the kernel's code object is paid
+  // here and not in the first plan.
+  if (ok && !d.warm) {
+    ok = hipMemsetAsync(d.x, 0, (size_t)mlp::F * sizeof(float), d.stream) == hipSuccess &&
+         mlp::launch_forward(d.params, d.x, 1, d.out, d.stream, 1) == hipSuccess &&
+         hipStreamSynchronize(d.stream) == hipSuccess;
+    if (ok) d.warm = true;
   }
   if (!ok) (void)hipGetLastError();
 }

@@ -489,7 +489,7 @@ class Server {
 
   int device_ = 0;
   DeviceInfo dev_;
-  int grid_[kBatchFamilies] = {0, 0, 0};
+  int grid_[kBatchFamilies] = {};
   int grid_max_ = 0;
   int max_batch_ = 64;
   Arena arena_;
