@@ -139,6 +139,10 @@ struct TiledArgs {
   FastDiv group_row_div;  // group * ntx
   FastDiv last_div;       // rows in the last (possibly partial) group
   int last_group_start;   // first tile index of the last group
+  // > 1: workgroup b starts at tile xcd_logical(b, grid, xcds), so each XCD
+  // walks one contiguous range of tiles (tiles that share 128-byte lines
+  // then meet in one XCD's L2); 1: workgroup b starts at tile b.
+  int xcds;
   OuterMap outer;
 };
 
@@ -163,6 +167,7 @@ struct CopyArgs {
   FastDiv ntx_div;
   FastDiv ntiles_div;
   int items;
+  int xcds;          // work order across XCDs, as in TiledArgs
   OuterMap outer;    // strides in words
 };
 

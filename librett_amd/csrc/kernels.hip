@@ -52,7 +52,8 @@ __global__ __launch_bounds__(256) void k_tiled(TiledArgs a, const void* __restri
   __shared__ E lds[Ops::LDS_ELEMS];
   const E* src = static_cast<const E*>(in);
   E* dst = static_cast<E*>(out);
-  for (uint32_t w = blockIdx.x; w < (uint32_t)a.items; w += gridDim.x) {
+  const uint32_t w0 = a.xcds > 1 ? (uint32_t)xcd_logical((int)blockIdx.x, (int)gridDim.x, a.xcds) : blockIdx.x;
+  for (uint32_t w = w0; w < (uint32_t)a.items; w += gridDim.x) {
     const TileCtx c = Ops::item(a, w);
     Ops::load(a, c, threadIdx.x, src, lds);
     __syncthreads();
@@ -68,7 +69,8 @@ __global__ __launch_bounds__(256) void k_copy(CopyArgs a, const void* __restrict
   using V = typename Ops::V;
   const V* src = static_cast<const V*>(in);
   V* dst = static_cast<V*>(out);
-  for (uint32_t w = blockIdx.x; w < (uint32_t)a.items; w += gridDim.x) {
+  const uint32_t w0 = a.xcds > 1 ? (uint32_t)xcd_logical((int)blockIdx.x, (int)gridDim.x, a.xcds) : blockIdx.x;
+  for (uint32_t w = w0; w < (uint32_t)a.items; w += gridDim.x) {
     const TileCtx c = Ops::item(a, w);
     Ops::run(a, c, threadIdx.x, src, dst);
   }

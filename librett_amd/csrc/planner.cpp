@@ -41,6 +41,7 @@ int tile_group() {
   return v;
 }
 
+
 long long prod(const int* d, int n) {
   long long v = 1;
   for (int i = 0; i < n; i++) v *= d[i];
@@ -332,13 +333,15 @@ std::string Candidate::describe() const {
   switch (method) {
     case kTiled:
       if (xgroup > 1 || ygroup > 1)
-        snprintf(buf, sizeof(buf), "%s tile %dx%d vec %d/%d groups %d/%d grid %d", n, tile_x, tile_y, vec_x, vec_y,
-                 xgroup, ygroup, grid_x);
+        snprintf(buf, sizeof(buf), "%s tile %dx%d vec %d/%d groups %d/%d grid %d%s", n, tile_x, tile_y, vec_x,
+                 vec_y, xgroup, ygroup, grid_x, tile_order(*this) ? " xcd" : "");
       else
-        snprintf(buf, sizeof(buf), "%s tile %dx%d vec %d/%d grid %d", n, tile_x, tile_y, vec_x, vec_y, grid_x);
+        snprintf(buf, sizeof(buf), "%s tile %dx%d vec %d/%d grid %d%s", n, tile_x, tile_y, vec_x, vec_y, grid_x,
+                 tile_order(*this) ? " xcd" : "");
       break;
     case kTiledCopy:
-      snprintf(buf, sizeof(buf), "%s word %dB tx %d grid %d", n, word, copy_tx, grid_x);
+      snprintf(buf, sizeof(buf), "%s word %dB tx %d grid %d%s", n, word, copy_tx, grid_x,
+               tile_order(*this) ? " xcd" : "");
       break;
     case kPacked:
     case kPackedSplit:
@@ -459,6 +462,7 @@ void add_tiled_plane(const Problem& r, const DeviceInfo& dev, int a_dims, int b_
     a.ntiles_div = make_fastdiv((uint32_t)a.ntiles);
     a.items = (int)std::min<long long>((long long)a.ntiles * vol_outer, 0x7fffffffLL);
     set_tile_group(a, nty, tile_group());
+    a.xcds = 1;
     a.outer = outer;
     c.block = 256;
     c.lds_bytes = c.tile_y * (c.tile_x + 1) * B;
@@ -548,6 +552,7 @@ void add_tiled_copy(const Problem& r, const DeviceInfo& dev, std::vector<Candida
     a.ntx_div = make_fastdiv((uint32_t)a.ntx);
     a.ntiles_div = make_fastdiv((uint32_t)a.ntiles);
     a.items = (int)std::min<long long>((long long)a.ntiles * vol_outer, 0x7fffffffLL);
+    a.xcds = 1;
     a.outer = outer;
     c.block = 256;
     c.lds_bytes = 0;
@@ -1214,9 +1219,48 @@ void model_packed(Candidate& c, const DeviceInfo& dev, const PackedSums& S) {
 
 // True when b is a's plan in the other work order (same sums).
 bool order_twins(const Candidate& a, const Candidate& b) {
+  if (a.method == kTiled || a.method == kTiledCopy)
+    return b.method == a.method && tile_order(a) != tile_order(b) && a.tile_x == b.tile_x &&
+           a.tile_y == b.tile_y && a.vec_x == b.vec_x && a.vec_y == b.vec_y && a.word == b.word &&
+           a.copy_tx == b.copy_tx && a.xgroup == b.xgroup && a.ygroup == b.ygroup;
   return (a.method == kPacked || a.method == kPackedSplit) && same_packed(a, b) && a.grid_x == b.grid_x &&
          a.grid_y == b.grid_y && a.packed.outer_per_block == b.packed.outer_per_block &&
          a.source == b.source && a.packed.order != b.packed.order;
+}
+
+// Every Tiled and TiledCopy plan in both tile orders across XCDs, the two
+// next to each other: dispatch order (workgroup b starts at tile b) and
+// XCD-contiguous order (each XCD walks one contiguous range of tiles, so
+// tiles sharing 128-byte lines meet in one L2). Which is faster depends on
+// the shape (measured on MI355X: +5-15% for Tiled planes whose rows are not
+// line-aligned, -8% on some small group planes), so both are candidates.
+// The analytic model cannot tell them apart; its tie goes to the first,
+// XCD-contiguous for Tiled and dispatch order for TiledCopy, the better
+// default of each on the seeded ~256 MiB cases.
+void add_order_twins(const DeviceInfo& dev, std::vector<Candidate>& out) {
+  const int xcds = std::max(1, dev.cu_count / 32);  // 32 CUs per XCD on gfx950
+  if (xcds < 2) return;
+  std::vector<Candidate> res;
+  res.reserve(out.size() * 2);
+  for (const Candidate& c : out) {
+    if (c.method != kTiled && c.method != kTiledCopy) {
+      res.push_back(c);
+      continue;
+    }
+    Candidate x = c;
+    if (x.method == kTiled)
+      x.tiled.xcds = xcds;
+    else
+      x.copy.xcds = xcds;
+    if (c.method == kTiled) {
+      res.push_back(x);
+      res.push_back(c);
+    } else {
+      res.push_back(c);
+      res.push_back(x);
+    }
+  }
+  out.swap(res);
 }
 
 }  // namespace
@@ -1298,6 +1342,7 @@ std::vector<Candidate> make_candidates(const Problem& full, const DeviceInfo& de
     add_tiled_copy(red, dev, out);
   else
     add_tiled(red, dev, out);
+  if (!dev.batch_mode) add_order_twins(dev, out);
   add_packed(full, "full", dev, out);
   if (red.rank != full.rank) add_packed(red, "reduced", dev, out);
   for (auto& c : out)
@@ -1313,10 +1358,14 @@ std::vector<Candidate> make_candidates(const Problem& full, const DeviceInfo& de
     Candidate& c = out[i];
     if (c.method == kPacked || c.method == kPackedSplit) {
       if (!twin[i]) sums[i] = packed_sums(c);
-    } else {
+    } else if (!twin[i]) {
       model_candidate(c, full, dev);
     }
   });
+  // A Tiled / TiledCopy twin has the same counts (the model does not see
+  // which XCD runs a tile).
+  for (size_t i = 0; i < n; i++)
+    if (twin[i] && (out[i].method == kTiled || out[i].method == kTiledCopy)) out[i].model = out[i - 1].model;
   for (size_t i = 0; i < n; i++) {
     Candidate& c = out[i];
     if (c.method != kPacked && c.method != kPackedSplit) continue;

@@ -110,6 +110,12 @@ struct Candidate {
   std::string describe() const;
 };
 
+// Tile order of a Tiled or TiledCopy candidate: 1 = XCD-contiguous, 0 =
+// dispatch order (0 for other methods).
+inline int tile_order(const Candidate& c) {
+  return (c.method == kTiled && c.tiled.xcds > 1) || (c.method == kTiledCopy && c.copy.xcds > 1) ? 1 : 0;
+}
+
 // All candidates for a problem. The list is empty only for invalid input.
 // Follows the method rules of createPlans (src/plan.cpp:702-719): Trivial
 // alone when the reduced rank is 1; TiledCopy when the reduced perm keeps

@@ -190,11 +190,12 @@ void selector_features(const Problem& full, const Problem& red, const Candidate&
   const double t_roof = std::max(1.0, (double)c.bytes) / 8.0e12;
   f[46] = lg(m.predicted_s / t_roof);
   f[47] = lg(m.t_stream / t_roof);
-  // Packed work order; for Tiled, the extra dims in the plane's x and y groups.
-  f[48] = (float)((c.method == kPacked || c.method == kPackedSplit) ? c.packed.order
-                  : c.method == kTiled                            ? (c.xgroup - 1) + (c.ygroup - 1)
-                                                                  : 0);
-  f[49] = std::strcmp(c.source, "full") == 0 ? 1.0f : 0.0f;
+  // Packed: work order and whether the plan is of the full (not the
+  // rank-reduced) problem. Tiled: the extra dims in the plane's x and y
+  // groups, and the tile order across XCDs; TiledCopy: its tile order.
+  const bool packed = c.method == kPacked || c.method == kPackedSplit;
+  f[48] = (float)(packed ? c.packed.order : c.method == kTiled ? (c.xgroup - 1) + (c.ygroup - 1) : 0);
+  f[49] = packed ? (std::strcmp(c.source, "full") == 0 ? 1.0f : 0.0f) : (float)tile_order(c);
 }
 
 uint64_t selector_generation() {
