@@ -68,6 +68,10 @@ __device__ __forceinline__ void dev_tiled(const BatchDesc& D, int lu, unsigned c
   E* lds = reinterpret_cast<E*>(lds_raw);
   for (int w = lu; w < D.items; w += D.groups) {
     const TileCtx c = Ops::item(D.a.tiled, (uint32_t)w);
+    if constexpr (G) {
+      Ops::prepare(D.a.tiled, c, threadIdx.x, lds);
+      __syncthreads();
+    }
     Ops::load(D.a.tiled, c, threadIdx.x, static_cast<const E*>(D.in), lds);
     __syncthreads();
     Ops::store(D.a.tiled, c, threadIdx.x, static_cast<E*>(D.out), lds);
@@ -200,6 +204,7 @@ void host_tiled(const BatchDesc& D, int lu, unsigned char* lds_raw) {
   E* lds = reinterpret_cast<E*>(lds_raw);
   for (int w = lu; w < D.items; w += D.groups) {
     const TileCtx c = Ops::item(D.a.tiled, (uint32_t)w);
+    for (int t = 0; t < 256; t++) Ops::prepare(D.a.tiled, c, t, lds);
     for (int t = 0; t < 256; t++) Ops::load(D.a.tiled, c, t, static_cast<const E*>(D.in), lds);
     for (int t = 0; t < 256; t++) Ops::store(D.a.tiled, c, t, static_cast<E*>(D.out), lds);
   }

@@ -113,9 +113,36 @@ struct TiledOps {
   static constexpr int WP = NT / LW;   // output rows (x) per pass
   static constexpr int NPW = TX / WP;  // write passes
   static constexpr int PITCH = TX + 1;
-  static constexpr int LDS_ELEMS = TY * PITCH;
+  // G: after the tile, TY input offsets of the tile's rows and TX output
+  // offsets of its columns (ints), written once per tile by prepare().
+  static constexpr int OFF_ELEMS = G ? ((TX + TY) * 4 + B - 1) / B : 0;
+  static constexpr int LDS_ELEMS = TY * PITCH + OFF_ELEMS;
   static_assert(TX % VX == 0 && NT % LR == 0 && TY % RP == 0 && NPR >= 1, "read shape");
   static_assert(TY % VY == 0 && NT % LW == 0 && TX % WP == 0 && NPW >= 1, "write shape");
+  static_assert(!G || TX + TY <= NT, "one thread per tile row and column");
+
+  __host__ __device__ static int* offsets(E* lds) { return reinterpret_cast<int*>(lds + TY * PITCH); }
+  __host__ __device__ static const int* offsets(const E* lds) {
+    return reinterpret_cast<const int*>(lds + TY * PITCH);
+  }
+
+  // G only, before load(): thread t < TY finds the input offset of tile row
+  // t, thread TY + i the output offset of tile column i (group digits, one
+  // magic-number division each), so the load and store loops read them from
+  // LDS instead of each thread dividing for every row it moves. A barrier
+  // must separate prepare() from load().
+  __host__ __device__ static void prepare(const TiledArgs& a, const TileCtx& c, int t, E* lds) {
+    if constexpr (G) {
+      int* o = offsets(lds);
+      if (t < TY) {
+        const int y = c.y0 + t;
+        o[t] = y < a.Y ? tiled_in_y(a, (uint32_t)y) : 0;
+      } else if (t < TY + TX) {
+        const int x = c.x0 + (t - TY);
+        o[t] = x < a.X ? tiled_out_x(a, (uint32_t)x) : 0;
+      }
+    }
+  }
 
   __host__ __device__ static TileCtx item(const TiledArgs& a, uint32_t w) {
     TileCtx c;
@@ -142,7 +169,7 @@ struct TiledOps {
   __host__ __device__ static void load(const TiledArgs& a, const TileCtx& c, int t,
                                        const E* __restrict__ in, E* lds) {
     VI r[NPR];
-    fetch(a, c, t, in, r);
+    fetch(a, c, t, in, r, lds);
     stash(a, c, t, r, lds);
   }
 
@@ -150,7 +177,7 @@ struct TiledOps {
   // registers into LDS (the pipelined kernel fetches the next tile between
   // them and the current tile's store phase).
   __host__ __device__ static void fetch(const TiledArgs& a, const TileCtx& c, int t, const E* __restrict__ in,
-                                        VI* r) {
+                                        VI* r, const E* lds) {
     const int lx = (t % LR) * VX;
     const int ly = t / LR;
     const int x = c.x0 + lx;
@@ -164,7 +191,7 @@ struct TiledOps {
 #pragma unroll
       for (int j = 0; j < NPR; j++) {
         const int y = c.y0 + ly + j * RP;
-        const int yoff = G ? tiled_in_y(a, (uint32_t)y) : y * a.in_ys;
+        const int yoff = G ? offsets(lds)[ly + j * RP] : y * a.in_ys;
         if (xin && y < a.Y) r[j] = ld(reinterpret_cast<const VI*>(in + (c.off_in + x + yoff)));
       }
     }
@@ -202,7 +229,7 @@ struct TiledOps {
 #pragma unroll
       for (int j = 0; j < NPW; j++) {
         const int x = c.x0 + lx + j * WP;
-        const int xoff = G ? tiled_out_x(a, (uint32_t)x) : x * a.out_xs;
+        const int xoff = G ? offsets(lds)[TY + lx + j * WP] : x * a.out_xs;
         if (yin && x < a.X) st(reinterpret_cast<VO*>(out + (c.off_out + y + xoff)), r[j]);
       }
     }

@@ -55,6 +55,10 @@ __global__ __launch_bounds__(256) void k_tiled(TiledArgs a, const void* __restri
   const uint32_t w0 = a.xcds > 1 ? (uint32_t)xcd_logical((int)blockIdx.x, (int)gridDim.x, a.xcds) : blockIdx.x;
   for (uint32_t w = w0; w < (uint32_t)a.items; w += gridDim.x) {
     const TileCtx c = Ops::item(a, w);
+    if constexpr (G) {
+      Ops::prepare(a, c, threadIdx.x, lds);
+      __syncthreads();
+    }
     Ops::load(a, c, threadIdx.x, src, lds);
     __syncthreads();
     Ops::store(a, c, threadIdx.x, dst, lds);
@@ -205,6 +209,7 @@ static void replay_tiled(const Candidate& c, const void* in, void* out) {
   E* dst = static_cast<E*>(out);
   for (uint32_t w = 0; w < (uint32_t)a.items; w++) {
     const TileCtx ctx = Ops::item(a, w);
+    for (int t = 0; t < Ops::NT; t++) Ops::prepare(a, ctx, t, lds.data());
     for (int t = 0; t < Ops::NT; t++) {
       replay_at(w, t);
       Ops::load(a, ctx, t, src, lds.data());

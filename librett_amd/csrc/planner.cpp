@@ -465,7 +465,8 @@ void add_tiled_plane(const Problem& r, const DeviceInfo& dev, int a_dims, int b_
     a.xcds = 1;
     a.outer = outer;
     c.block = 256;
-    c.lds_bytes = c.tile_y * (c.tile_x + 1) * B;
+    // Group planes keep the tile's row and column offsets after the tile.
+    c.lds_bytes = c.tile_y * (c.tile_x + 1) * B + (general ? ((c.tile_x + c.tile_y) * 4 + B - 1) / B * B : 0);
     c.grid_x = std::max(1, std::min(a.items, dev.cu_count * grid_per_cu()));
     c.bytes = 2 * r.vol * B;
     c.split.sizeMm = a_dims;
