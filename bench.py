@@ -82,6 +82,11 @@ def main():
                          "'bench.py --steps 1 --warmup 0 --no-cpu' (RUN_LOG: that run's stderr, for the plans) "
                          "into HBM traffic per named case; no GPU needed")
     ap.add_argument("--pmc-json", default=None, help="with --pmc-summary: also write the summary as JSON here")
+    ap.add_argument("--pmc-requests", nargs=2, metavar=("READ_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc runs of the raw TCC/EA request counters (reads: "
+                         "TCC_EA0_RDREQ_sum TCC_BUBBLE_sum TCC_EA0_RDREQ_32B_sum TCC_EA0_RDREQ_DRAM_sum; writes: "
+                         "TCC_EA0_WRREQ_sum TCC_EA0_WRREQ_64B_sum TCC_EA0_WRREQ_DRAM_sum) of "
+                         "'bench.py --steps 1 --warmup 0 --no-cpu' and exit")
     ap.add_argument("--requests", type=int, default=20000, help="serving workload: arrivals per rate")
     ap.add_argument("--arena-gb", type=float, default=64.0, help="serving workload: device arena budget (GiB)")
     ap.add_argument("--slots", type=int, default=8, help="serving workload: micro-batches in flight")
@@ -91,6 +96,8 @@ def main():
     args = ap.parse_args()
     if args.pmc_summary:
         return pmc_summary(*args.pmc_summary, json_path=args.pmc_json)
+    if args.pmc_requests:
+        return pmc_requests(*args.pmc_requests)
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X and has no "
               "multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -828,6 +835,83 @@ def pmc_summary(fetch_dir, write_dir, run_log, json_path=None):
                                                                                    out["write_factor"]))
             fh.write(",\n".join(json.dumps(c) for c in out["cases"]))
             fh.write("\n]}\n")
+
+
+def _pmc_table(d):
+    """{dispatch id: (kernel, grid, {counter: value})} of one counter run."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no counter_collection.csv under %s" % d)
+    t = {}
+    for r in csv.DictReader(open(files[0])):
+        k = int(r["Dispatch_Id"])
+        ent = t.setdefault(k, (r["Kernel_Name"], int(r["Grid_Size"]), {}))
+        ent[2][r["Counter_Name"]] = ent[2].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return t
+
+
+def pmc_requests(read_dir, write_dir):
+    """Checks what FETCH_SIZE and WRITE_SIZE count on this GPU against the raw
+    TCC/EA request counters they are built from. FETCH_SIZE charges 128 B per
+    TCC_BUBBLE, 32 B per TCC_EA0_RDREQ_32B and 64 B per other TCC_EA0_RDREQ;
+    WRITE_SIZE 64 B per TCC_EA0_WRREQ_64B and 32 B per other TCC_EA0_WRREQ.
+    For the streaming copy kernel the bytes are known (Grid_Size 16-byte words
+    each way), so bytes / requests gives the request size the hardware
+    actually moves."""
+    rd, wr = _pmc_table(read_dir), _pmc_table(write_dir)
+    lines = ["| kernel | dispatches | MB read (alg.) | RDREQ | BUBBLE | RDREQ_32B | RDREQ_DRAM | B per RDREQ | "
+             "FETCH_SIZE formula / bytes | MB written (alg.) | WRREQ | WRREQ_64B | WRREQ_DRAM | B per WRREQ | "
+             "WRITE_SIZE formula / bytes |", "|---" * 15 + "|"]
+    seen = {}
+    # The first transpose dispatches are the named cases in order (one step,
+    # no warm-up); their algorithmic bytes per side are vol * elem.
+    case_bytes = [float(np.prod(d)) * e for d, _, e in CASES]
+    ntr = 0
+    for k in sorted(rd):
+        name, grid, c = rd[k]
+        w = wr.get(k, (name, grid, {}))[2]
+        short = name.split("(")[0].replace("void ", "")
+        alg = None
+        if "k_devcopy" in name:
+            alg = grid * 16.0
+        elif any(t in name for t in ("k_tiled", "k_copy<", "k_packed")):
+            if ntr < len(case_bytes):
+                alg = case_bytes[ntr]
+                short = "%s %s" % (short, "x".join(str(x) for x in CASES[ntr][0]))
+            ntr += 1
+        ent = seen.setdefault(short, {"n": 0, "alg": 0.0, "rd": 0.0, "bub": 0.0, "r32": 0.0, "rdram": 0.0,
+                                      "wr": 0.0, "w64": 0.0, "wdram": 0.0, "known": alg is not None})
+        ent["n"] += 1
+        ent["alg"] += alg or 0.0
+        ent["rd"] += c.get("TCC_EA0_RDREQ_sum", 0.0)
+        ent["bub"] += c.get("TCC_BUBBLE_sum", 0.0)
+        ent["r32"] += c.get("TCC_EA0_RDREQ_32B_sum", 0.0)
+        ent["rdram"] += c.get("TCC_EA0_RDREQ_DRAM_sum", 0.0)
+        ent["wr"] += w.get("TCC_EA0_WRREQ_sum", 0.0)
+        ent["w64"] += w.get("TCC_EA0_WRREQ_64B_sum", 0.0)
+        ent["wdram"] += w.get("TCC_EA0_WRREQ_DRAM_sum", 0.0)
+    for short, e in seen.items():
+        if not e["known"] or e["rd"] <= 0 or e["wr"] <= 0:
+            continue
+        fetch = e["bub"] * 128 + (e["rd"] - e["bub"] - e["r32"]) * 64 + e["r32"] * 32
+        write = (e["wr"] - e["w64"]) * 32 + e["w64"] * 64
+        lines.append("| `%s` | %d | %.1f | %.0f | %.0f | %.0f | %.0f | %.1f | %.4f | %.1f | %.0f | %.0f | %.0f | %.1f | "
+                     "%.4f |" % (short, e["n"], e["alg"] / 1e6, e["rd"], e["bub"], e["r32"], e["rdram"],
+                                 e["alg"] / e["rd"], fetch / e["alg"], e["alg"] / 1e6, e["wr"], e["w64"],
+                                 e["wdram"], e["alg"] / e["wr"], write / e["alg"]))
+    lines += ["", "Every dispatch, all kernels (requests, not bytes):", "",
+              "| dispatch | kernel | RDREQ | BUBBLE | RDREQ_32B | RDREQ_DRAM | WRREQ | WRREQ_64B | WRREQ_DRAM |",
+              "|---|---|---|---|---|---|---|---|---|"]
+    for k in sorted(rd):
+        name, grid, c = rd[k]
+        w = wr.get(k, (name, grid, {}))[2]
+        lines.append("| %d | `%s` | %.0f | %.0f | %.0f | %.0f | %.0f | %.0f | %.0f |" % (
+            k, name.split("(")[0].replace("void ", "")[:60], c.get("TCC_EA0_RDREQ_sum", 0), c.get("TCC_BUBBLE_sum", 0),
+            c.get("TCC_EA0_RDREQ_32B_sum", 0), c.get("TCC_EA0_RDREQ_DRAM_sum", 0), w.get("TCC_EA0_WRREQ_sum", 0),
+            w.get("TCC_EA0_WRREQ_64B_sum", 0), w.get("TCC_EA0_WRREQ_DRAM_sum", 0)))
+    print("\n".join(lines))
 
 
 def measured_traffic(dim, perm, elem):

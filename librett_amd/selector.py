@@ -351,6 +351,7 @@ def main(argv=None):
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--heldout-out", default=None, help="train: also save the held-out records here")
     ap.add_argument("--models", type=int, default=1,
                     help="build / train: networks in the ensemble (seeds --seed, --seed + 1, ...)")
     a = ap.parse_args(argv)
@@ -408,6 +409,8 @@ def main(argv=None):
             s_ho, a_ho = evaluate(ho, scores(p, mean, std, ho, stream))
             rep["heldout_problems"] = len(ho)
             rep["heldout"] = {"selector": summary(s_ho), "analytic": summary(a_ho)}
+            if a.heldout_out:
+                save_records(a.heldout_out, ho)
         print(json.dumps(rep))
         if a.report:
             with open(a.report, "w") as fh:
