@@ -113,9 +113,28 @@ def random_problem(rng, rank_lo=2, rank_hi=7, bytes_lo=1 << 20, bytes_hi=512 << 
     return dims, perm, elem
 
 
-def problems(n, seed, **kw):
+def pow2_problem(rng, rank_lo=2, rank_hi=6, bytes_lo=1 << 20, bytes_hi=512 << 20, elems=(4, 8, 16)):
+    """Like random_problem, but every dim a power of two (>= 2): extents whose
+    rows fill whole 128-byte lines, which random dims almost never give."""
+    elem = int(rng.choice(elems))
+    rank = int(rng.integers(rank_lo, rank_hi + 1))
+    total = int(round(np.log2(float(np.exp(rng.uniform(np.log(bytes_lo), np.log(bytes_hi)))) / elem)))
+    total = max(rank, total)
+    w = rng.dirichlet(np.ones(rank))
+    ks = [max(1, int(round(total * wi))) for wi in w]
+    while sum(ks) > total and max(ks) > 1:
+        ks[int(np.argmax(ks))] -= 1
+    dims = [1 << k for k in ks]
+    perm = [int(v) for v in rng.permutation(rank)]
+    return dims, perm, elem
+
+
+def problems(n, seed, family="random", **kw):
+    """n seeded problems of one family: "random" (random_problem) or "pow2"
+    (pow2_problem)."""
     rng = np.random.default_rng(seed)
-    return [random_problem(rng, **kw) for _ in range(n)]
+    gen = pow2_problem if family == "pow2" else random_problem
+    return [gen(rng, **kw) for _ in range(n)]
 
 
 # ---------------------------------------------------------------------------
@@ -158,15 +177,21 @@ def save_records(path, recs, with_features=True):
 
 
 def load_records(path):
-    z = np.load(path, allow_pickle=False)
-    off = z["offsets"]
-    has_f = "feats" in z.files
+    # Each member of an .npz is read (and decompressed) once: indexing the
+    # file object per record would decompress the whole array every time.
+    with np.load(path, allow_pickle=False) as z:
+        off = z["offsets"]
+        feats = z["feats"] if "feats" in z.files else None
+        times = z["times"].astype(np.float64)
+        problems = z["problems"]
+        elem = z["elem"]
+        choice = z["model_choice"]
     recs = []
     for i in range(len(off) - 1):
         a, b = off[i], off[i + 1]
-        d, p = json.loads(str(z["problems"][i]))
-        recs.append({"dims": d, "perm": p, "elem": int(z["elem"][i]), "feats": z["feats"][a:b] if has_f else None,
-                     "times": z["times"][a:b].astype(np.float64), "model_choice": int(z["model_choice"][i])})
+        d, p = json.loads(str(problems[i]))
+        recs.append({"dims": d, "perm": p, "elem": int(elem[i]), "feats": feats[a:b] if feats is not None else None,
+                     "times": times[a:b], "model_choice": int(choice[i])})
     return recs
 
 
@@ -182,7 +207,7 @@ def attach_features(recs, log=None):
             r = dict(r)
             r["feats"] = feats
             out.append(r)
-        if log and time.time() - last > 20:
+        if log and time.time() - last > 5:
             last = time.time()
             log("features %d/%d (%.0f s)" % (k + 1, len(recs), last - t0))
     if log:
@@ -346,6 +371,8 @@ def main(argv=None):
     ap.add_argument("--offset", type=int, default=0, help="collect: first problem of the list to time")
     ap.add_argument("--problems", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--family", choices=["random", "pow2"], default="random",
+                    help="collect: problem family (random dims, or powers of two)")
     ap.add_argument("--out", default="selector_data.npz")
     ap.add_argument("--data", nargs="+", default=None)
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
@@ -360,6 +387,7 @@ def main(argv=None):
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
+    log("selector %s: start" % a.cmd)
     stream = hip.stream_create()
     if a.cmd == "build":
         A, B = _buffers(512 << 20)
@@ -385,13 +413,14 @@ def main(argv=None):
             save_records(a.out, ho)
     elif a.cmd == "collect":
         A, B = _buffers(512 << 20)
-        probs = problems(a.total or a.problems, a.seed)[a.offset:a.offset + a.problems]
+        probs = problems(a.total or a.problems, a.seed, a.family)[a.offset:a.offset + a.problems]
         recs = collect(probs, stream, A.ptr, B.ptr, log)
         save_records(a.out, recs, with_features=False)
     elif a.cmd == "train":
         recs = []
         for path in a.data:
             recs += load_records(path)
+        log("loaded %d records from %d files" % (len(recs), len(a.data)))
         if any(r["feats"] is None for r in recs):
             recs = attach_features(recs, log)
         t1 = time.time()

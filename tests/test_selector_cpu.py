@@ -217,6 +217,28 @@ def test_records_roundtrip_without_features(tmp_path):
     assert np.isnan(y[1]) and np.all(np.isfinite(y[[0, 2]]))
 
 
+def test_records_roundtrip_with_features_many(tmp_path):
+    # Thousands of records load in well under a second: every member of the
+    # file is read once, not once per record.
+    import time
+    rng = np.random.default_rng(3)
+    recs = []
+    for i in range(3000):
+        n = int(rng.integers(1, 40))
+        recs.append({"dims": [2 + i % 7, 3], "perm": [1, 0], "elem": 8,
+                     "feats": rng.standard_normal((n, M.F)).astype(np.float32),
+                     "times": rng.uniform(1e-5, 1e-3, n), "model_choice": int(rng.integers(0, n))})
+    path = str(tmp_path / "f.npz")
+    S.save_records(path, recs)
+    t = time.time()
+    back = S.load_records(path)
+    assert time.time() - t < 5.0
+    assert len(back) == len(recs)
+    for r, b in zip(recs[::97], back[::97]):
+        assert np.array_equal(r["feats"], b["feats"]) and np.allclose(r["times"], b["times"], rtol=1e-6)
+        assert r["model_choice"] == b["model_choice"] and r["dims"] == b["dims"]
+
+
 def test_unloading_first_keeps_analytic_model(tmp_path):
     # In a fresh process, clearing the weights before anything else must not
     # be undone by the first-use load of the default weights file.
