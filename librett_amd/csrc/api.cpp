@@ -185,9 +185,6 @@ hipError_t run_plan(const PlanObj& p, const void* in, void* out) {
 // Times one candidate: one warm-up launch, then `reps` launches each after an
 // output memset, on the plan's stream. Returns the median in seconds, or a
 // negative value on failure.
-// Runs timed for a candidate whose first run took under 50 us.
-constexpr int kShortRunReps = 15;
-
 double time_candidate(const Candidate& c, hipStream_t stream, const void* in, void* out, int reps) {
   if (prepare_kernel(c) != 0) return -1;
   hipEvent_t e0, e1;
@@ -200,9 +197,6 @@ double time_candidate(const Candidate& c, hipStream_t stream, const void* in, vo
   bool ok = launch_candidate(c, in, out, stream) == hipSuccess;
   const size_t nbytes = (size_t)(c.bytes / 2);
   for (int r = 0; ok && r < reps; r++) {
-    // Runs of a few microseconds vary by their launch and ramp-up alone, so
-    // short candidates get more runs (cheap at that size) before the median.
-    if (r == 1 && !ts.empty() && ts[0] < 0.05f) reps = std::max(reps, kShortRunReps);
     ok = hipMemsetAsync(out, 0xFF, nbytes, stream) == hipSuccess;
     ok = ok && hipEventRecord(e0, stream) == hipSuccess;
     ok = ok && launch_candidate(c, in, out, stream) == hipSuccess;
@@ -263,8 +257,7 @@ void fill_info(const Candidate& c, double measured, librett_c_plan_info* info) {
 
 // librettPlanMeasure's timing of every candidate (src/librett.cpp:224-343
 // times each candidate once; here each gets a warm-up and the median of
-// LIBRETT_MEASURE_REPS runs, at least 15 when a run takes under 50 us).
-// Times what librettExecute would run on these
+// LIBRETT_MEASURE_REPS runs). Times what librettExecute would run on these
 // pointers; a Trivial candidate is exclusive and ends the list. Negative
 // entries are candidates that failed to run.
 std::vector<double> measure_all(const Problem& full, const std::vector<Candidate>& cands, const DeviceInfo& dev,
