@@ -151,6 +151,19 @@ int load_weights(const char* path) {
 
 }  // namespace
 
+namespace {
+
+int line_alignment(long long bytes) {
+  int k = 0;
+  while (k < 7 && bytes > 0 && (bytes & 1) == 0) {
+    bytes >>= 1;
+    k++;
+  }
+  return k;
+}
+
+}  // namespace
+
 void selector_features(const Problem& full, const Problem& red, const Candidate& c, float* f) {
   for (int i = 0; i < mlp::F; i++) f[i] = 0.0f;
   f[0] = (float)red.rank;
@@ -160,7 +173,12 @@ void selector_features(const Problem& full, const Problem& red, const Candidate&
   }
   f[15] = lg(full.elem);
   f[16] = lg((double)full.vol * full.elem);
-  f[17] = (float)full.rank;
+  // How well the contiguous runs fill 128-byte lines: log2 of the largest
+  // power of two (at most 128) dividing the byte length of the input's and
+  // of the output's leading run, the smaller of the two (7: every row starts
+  // on a line, as in power-of-two shapes; 2 or 3: rows straddle lines).
+  f[17] = (float)std::min(line_alignment((long long)red.dim[0] * full.elem),
+                          line_alignment((long long)red.dim[red.perm[0]] * full.elem));
   if (c.method >= kTrivial && c.method <= kTiledCopy) f[18 + (c.method - kTrivial)] = 1.0f;
   f[23] = lg(c.tile_x);
   f[24] = lg(c.tile_y);
