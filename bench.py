@@ -340,6 +340,24 @@ def bench_large_random(args, stream):
             ts.append(e1.elapsed_ms(e0) * 1e-3)
         return sorted(ts)[len(ts) // 2]
 
+    def timed_pair(f1, f2, n):
+        # Both warmed up, then their runs alternate (f1 first on even rounds,
+        # f2 first on odd ones), so neither is timed in a systematically
+        # different state of the device; medians of each.
+        f1()
+        f2()
+        hip.stream_sync(stream)
+        t1, t2 = [], []
+        for i in range(n):
+            order = ((f1, t1), (f2, t2)) if i % 2 == 0 else ((f2, t2), (f1, t1))
+            for fn, ts in order:
+                e0.record(stream)
+                fn()
+                e1.record(stream)
+                e1.synchronize()
+                ts.append(e1.elapsed_ms(e0) * 1e-3)
+        return sorted(t1)[len(t1) // 2], sorted(t2)[len(t2) // 2]
+
     rows = []
     steps = max(3, args.steps // 2)
     for elem in (4, 8):
@@ -352,9 +370,11 @@ def bench_large_random(args, stream):
                 r2, h2 = L.librettPlanMeasure(len(d), d, p, elem, stream, A.ptr, B.ptr)
                 if r1 != L.LIBRETT_SUCCESS or r2 != L.LIBRETT_SUCCESS:
                     raise SystemExit("plan failed for %s %s" % (d, p))
-                t_plan = timed(lambda: L.librettExecute(h1, A.ptr, B.ptr), steps)
+                t_plan, t_meas = timed_pair(lambda: L.librettExecute(h1, A.ptr, B.ptr),
+                                            lambda: L.librettExecute(h2, A.ptr, B.ptr), steps)
+                L.librettExecute(h1, A.ptr, B.ptr)
                 bad1 = L.check_transpose(B.ptr, d, p, elem, stream)[0]
-                t_meas = timed(lambda: L.librettExecute(h2, A.ptr, B.ptr), steps)
+                L.librettExecute(h2, A.ptr, B.ptr)
                 bad2 = L.check_transpose(B.ptr, d, p, elem, stream)[0]
                 t_d2d = timed(lambda: hip.memcpy_d2d_async(B.ptr, A.ptr, n, stream), steps)
                 if bad1 or bad2:
