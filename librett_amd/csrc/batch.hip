@@ -49,6 +49,23 @@ struct BatchShape {
 const BatchShape kBatchShapes[] = {LRT_BATCH_TABLE(LRT_BATCH_ROW)};
 #undef LRT_BATCH_ROW
 
+// The problem owning unit u, searching forward from problem p (u only grows):
+// the smallest q >= p with start[q + 1] > u. Each lane of the wave tests one
+// problem, so one coalesced load and a ballot cover 64 problems (a scalar
+// walk would take one dependent load per problem skipped, and a workgroup
+// that strides by the grid skips many small problems per unit).
+// start[nprob] = total > u ends the search inside the array.
+__device__ __forceinline__ int batch_advance(const int* __restrict__ start, int nprob, int p, int u) {
+  if (start[p + 1] > u) return p;
+  const int lane = (int)(threadIdx.x & 63);
+  for (int base = p + 1;; base += 64) {
+    const int q = base + lane;
+    const bool hit = q >= nprob || start[q + 1] > u;
+    const unsigned long long m = __ballot(hit);
+    if (m) return base + __ffsll((long long)m) - 1;
+  }
+}
+
 __device__ __forceinline__ int batch_find(const int* __restrict__ start, int nprob, int u) {
   int lo = 0, hi = nprob;  // start[lo] <= u < start[hi]
   while (hi - lo > 1) {
@@ -171,7 +188,7 @@ __device__ __forceinline__ void dev_unit(const BatchDesc& D, int lu, unsigned ch
 // Persistent kernel of one family over `total` units of `nprob`
 // descriptors: workgroup g takes units g, g + G, g + 2G, ...
 template <int FAM>
-__global__ __launch_bounds__(256) void k_batch(const BatchDesc* __restrict__ descs, const int* __restrict__ start,
+__global__ __launch_bounds__(256, FAM == kFamPacked ? 6 : 1) void k_batch(const BatchDesc* __restrict__ descs, const int* __restrict__ start,
                                                int nprob, int total) {
   // TiledCopy moves data through registers only.
   __shared__ __attribute__((aligned(16))) unsigned char lds[FAM == kFamCopy ? 16 : kBatchLds];
@@ -179,7 +196,7 @@ __global__ __launch_bounds__(256) void k_batch(const BatchDesc* __restrict__ des
   if (u >= total) return;
   int p = batch_find(start, nprob, u);
   for (; u < total; u += gridDim.x) {
-    while (start[p + 1] <= u) p++;
+    p = batch_advance(start, nprob, p, u);
     dev_unit<FAM>(descs[p], u - start[p], lds);
   }
 }
