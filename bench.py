@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--arena-gb", type=float, default=64.0, help="serving workload: device arena budget (GiB)")
     ap.add_argument("--slots", type=int, default=8, help="serving workload: micro-batches in flight")
     ap.add_argument("--max-batch", type=int, default=64, help="serving workload: requests per micro-batch")
+    ap.add_argument("--clients", type=int, default=1, help="serving workload: client threads submitting requests")
     ap.add_argument("--problems", type=int, default=4096, help="batched workload: number of problems")
     ap.add_argument("--tune", type=int, default=3, help="batched workload: autotuner candidates per problem")
     args = ap.parse_args()
@@ -664,8 +665,12 @@ def bench_serving(args, stream):
     pool = hip.DeviceBuffer(pool_bytes)
     hip.memset(pool.ptr, 5, pool_bytes)
     srv = L.Server(int(args.arena_gb * (1 << 30)), slots=args.slots, max_batch=args.max_batch)
+    # One client thread submits by default: its admission and submission take
+    # a few microseconds a request, so on a slow host the request rate, not
+    # the device, can bound saturation (more client threads measured slower:
+    # they contend in the submission path).
     kw = dict(catalog=1024, min_bytes=4096, max_bytes=64 << 20, rank_lo=2, rank_hi=6, elems=(4, 8, 16),
-              inputs_in_arena=True)
+              inputs_in_arena=True, clients=args.clients)
     # Capacity: a rate far above what the device sustains (closed-loop saturation).
     srv.run_poisson(pool.ptr, pool_bytes, qps=1e7, requests=2000, seed=1, **kw)  # warm-up
     srv.reset_stats()
@@ -685,7 +690,7 @@ def bench_serving(args, stream):
         log("load %.2f: %8.0f req/s  %7.1f GB/s  p50 %8.1f us  p99 %8.1f us  batch %.1f  stalls %d" % (
             frac, frac * cap_qps, st["gbs"], st["p50_us"], st["p99_us"], st["mean_batch"], st["admission_stalls"]))
     # New shape per request: plans are made on arrival by 8 client threads.
-    fresh = dict(kw, fresh=True, clients=8)
+    fresh = dict(kw, fresh=True, clients=max(8, args.clients))
     srv.reset_stats()
     fsat = srv.run_poisson(pool.ptr, pool_bytes, qps=1e7, requests=args.requests, seed=5, **fresh)
     fcap = fsat["gbs"] * 1e9 / max(1.0, fsat["bytes"] / max(1, fsat["completed"]))
@@ -753,15 +758,16 @@ def bench_serving(args, stream):
         "dtype": "f32/f64/c128 (opaque moves)",
         "data": "synthetic",
         "config": {"workload": "serving: Poisson arrivals, 4 KiB-64 MiB log-uniform, rank 2-6, f32/f64/c128, "
-                               "%g GiB arena gating input + output bytes, LRU plan cache" % args.arena_gb,
-                   "slots": args.slots, "max_batch": args.max_batch},
+                               "%g GiB arena gating input + output bytes, LRU plan cache, %d client threads" % (
+                                   args.arena_gb, args.clients),
+                   "slots": args.slots, "max_batch": args.max_batch, "clients": args.clients},
         "saturation": {"gbs": round(sat["gbs"], 1), "req_per_s": round(cap_qps), "p50_us": round(sat["p50_us"], 1),
                        "p99_us": round(sat["p99_us"], 1), "mean_batch": round(sat["mean_batch"], 2),
                        "shapes": "catalog of 1024, planned up front"},
         "load_sweep": rates,
         "fresh_shapes": {"saturation_gbs": round(fsat["gbs"], 1), "req_per_s": round(fcap),
                          "cold_plan_us_mean": round(plan_us, 1), "plan_misses": fsat["plan_misses"],
-                         "client_threads": 8, "half_load_p50_us": round(fhalf["p50_us"], 1),
+                         "client_threads": max(8, args.clients), "half_load_p50_us": round(fhalf["p50_us"], 1),
                          "half_load_p99_us": round(fhalf["p99_us"], 1), "half_load_gbs": round(fhalf["gbs"], 1)},
         "per_call_loop_gbs": round(loop_gbs, 1),
         "speedup_vs_per_call": round(sat["gbs"] / loop_gbs, 2),
