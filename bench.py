@@ -310,7 +310,7 @@ def bench_named(args, stream):
             "avg_launch_ms": round(dom_ms / dom_launches, 4),
         },
         "cpu_baseline": cpu,
-        "reference_hip_build": "not measured: the reference sources are not on the GPU box (no LIBRETT_REF_DIR)",
+        "reference_hip_build": "not measured: the reference sources are not available on this machine (no LIBRETT_REF_DIR)",
         "plan_ms": {"%s %s %dB" % (tuple(c["dim"]), tuple(c["perm"]), c["elem"]): round(c["plan_s"] * 1e3, 2)
                     for c in cases},
         "cases": per_case,
