@@ -312,10 +312,23 @@ bool batch_copy_available(int word, int tx) {
   return false;
 }
 
+namespace {
+// Bit nreg of the entry for elem / 4 (elements of 4, 8 and 16 bytes): the
+// planner asks for every split it tries, so the table is indexed once.
+struct PackedShapeBits {
+  unsigned bits[5] = {0, 0, 0, 0, 0};
+  PackedShapeBits() {
+    for (const auto& s : kBatchShapes)
+      if (s.method == kPacked && (s.b == 4 || s.b == 8 || s.b == 16) && s.p1 >= 0 && s.p1 < 32)
+        bits[s.b / 4] |= 1u << s.p1;
+  }
+};
+}  // namespace
+
 bool batch_packed_available(int elem, int nreg) {
-  for (const auto& s : kBatchShapes)
-    if (s.method == kPacked && s.b == elem && s.p1 == nreg) return true;
-  return false;
+  static const PackedShapeBits t;
+  if ((elem != 4 && elem != 8 && elem != 16) || nreg < 0 || nreg >= 32) return false;
+  return (t.bits[elem / 4] >> nreg) & 1u;
 }
 
 int batch_variant(const Candidate& c) {

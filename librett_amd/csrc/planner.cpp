@@ -998,31 +998,38 @@ void model_copy(Candidate& c) {
 void staged_runs(const StagedShape& s, bool output_side, std::vector<std::pair<long long, int>>& runs) {
   runs.clear();
   const int n = s.n;
+  int dd[kMaxMmk];
+  long long ct[kMaxMmk];
+  for (int d = 0; d < n; d++) {
+    dd[d] = output_side ? (int)s.dout[d].d : (int)s.din[d].d;
+    ct[d] = output_side ? s.ct_out[d] : s.ct_in[d];
+  }
+  // A staged shape is a sub-box of a column-major array walked in that
+  // array's own order, so consecutive elements are adjacent exactly within
+  // the leading dims whose strides make them contiguous (dims of extent 1
+  // take no step): each run is one block of those dims. The blocks are
+  // listed by an odometer over the remaining dims, one step per run.
+  long long len = 1;
+  int k = 0;
+  while (k < n && (dd[k] == 1 || ct[k] == len)) {
+    len *= dd[k];
+    k++;
+  }
+  const long long nruns = len > 0 ? (long long)s.vol / len : 0;
+  runs.reserve((size_t)nruns);
   int dig[kMaxMmk] = {0};
   long long off = 0;
-  long long run_start = 0;
-  int run_len = 0;
-  long long expect = 0;
-  for (int p = 0; p < s.vol; p++) {
-    if (run_len > 0 && off == expect) {
-      run_len++;
-    } else {
-      if (run_len > 0) runs.push_back({run_start, run_len});
-      run_start = off;
-      run_len = 1;
-    }
-    expect = off + 1;
-    for (int d = 0; d < n; d++) {
-      const int dd = output_side ? (int)s.dout[d].d : (int)s.din[d].d;
-      const int ct = output_side ? s.ct_out[d] : s.ct_in[d];
-      dig[d]++;
-      off += ct;
-      if (dig[d] < dd) break;
-      off -= (long long)dd * ct;
+  for (long long r = 0; r < nruns; r++) {
+    runs.push_back({off, (int)len});
+    for (int d = k; d < n; d++) {
+      if (++dig[d] < dd[d]) {
+        off += ct[d];
+        break;
+      }
+      off -= (long long)(dd[d] - 1) * ct[d];
       dig[d] = 0;
     }
   }
-  if (run_len > 0) runs.push_back({run_start, run_len});
 }
 
 namespace {
