@@ -352,25 +352,37 @@ class Server {
     return k;
   }
 
-  // Fills a slot with queued requests and launches them. Called with mu_ held.
-  bool fill_and_launch(Slot& s) {
-    std::vector<std::shared_ptr<Request>> take;
+  // Fills a slot with queued requests and launches them. Called with mu_
+  // held (through `lock`); the requests are taken off the queue under the
+  // lock, but describing them, the uploads and the launches run without it,
+  // so clients can keep submitting meanwhile. Only the dispatcher touches a
+  // slot. Requests that do not fit the slot go back to the front of the
+  // queue in their order.
+  bool fill_and_launch(Slot& s, std::unique_lock<std::mutex>& lock) {
+    std::vector<std::shared_ptr<Request>> cand;
+    while (!pending_.empty() && (int)cand.size() < max_batch_) {
+      cand.push_back(pending_.front());
+      pending_.pop_front();
+    }
+    if (cand.empty()) return false;
+    for (auto& r : cand) r->status = 1;
+    lock.unlock();
+
+    std::vector<std::shared_ptr<Request>> take, failed, back;
     int ndesc = 0;
     std::vector<BatchDesc> ds, fs;
     std::vector<long long> offs;
     std::vector<BatchDesc> use;
-    while (!pending_.empty() && (int)take.size() < max_batch_) {
-      std::shared_ptr<Request> r = pending_.front();
+    for (size_t i = 0; i < cand.size(); i++) {
+      const std::shared_ptr<Request>& r = cand[i];
       if (!describe_problem(r->prob, r->plan, ds, fs, offs)) {
-        pending_.pop_front();
-        r->status = -1;
-        failed_++;
-        live_.erase(r->id);
-        done_cv_.notify_all();
+        failed.push_back(r);
         continue;
       }
-      if (ndesc + (int)ds.size() > s.cap) break;
-      pending_.pop_front();
+      if (ndesc + (int)ds.size() > s.cap) {
+        back.assign(cand.begin() + (long)i, cand.end());
+        break;
+      }
       const int align = batch_alignment(r->prob, r->plan);
       for (size_t k = 0; k < ds.size(); k++) {
         const char* in = (const char*)r->in + offs[k];
@@ -382,38 +394,44 @@ class Server {
         use.push_back(d);
         ndesc++;
       }
-      r->status = 1;
       take.push_back(r);
     }
-    if (take.empty()) return false;
-    long long bytes = 0;
-    for (auto& r : take) bytes += 2LL * r->prob.vol * r->prob.elem;
-    const long long unit_bytes = batch_unit_bytes(bytes, grid_max_);
-    for (int k = 0; k < ndesc; k++) batch_regroup(use[k], unit_bytes);
-    // One launch per kernel family, in family order on the slot's stream.
-    BatchLayout L;
-    batch_layout(use, L, nullptr);
-    bool fits = true;
-    for (int f = 0; f < kBatchFamilies; f++) fits = fits && L.total[f] < (1LL << 31);
-    for (int k = 0; k < ndesc; k++) s.h_desc[k] = use[k];
-    for (size_t k = 0; k < L.start.size(); k++) s.h_start[k] = L.start[k];
-    bool ok = fits &&
-              hipMemcpyAsync(s.d_desc, s.h_desc, ndesc * sizeof(BatchDesc), hipMemcpyHostToDevice, s.stream) ==
-                  hipSuccess &&
-              hipMemcpyAsync(s.d_start, s.h_start, L.start.size() * sizeof(int), hipMemcpyHostToDevice, s.stream) ==
-                  hipSuccess &&
-              launch_batch_all(s.d_desc, s.d_start, L, grid_, s.stream) == hipSuccess &&
-              hipEventRecord(s.done, s.stream) == hipSuccess;
-    if (!ok) {
-      (void)hipGetLastError();
-      for (auto& r : take) {
-        r->status = -1;
-        failed_++;
-        live_.erase(r->id);
-      }
-      done_cv_.notify_all();
-      return false;
+    bool ok = true;
+    if (!take.empty()) {
+      long long bytes = 0;
+      for (auto& r : take) bytes += 2LL * r->prob.vol * r->prob.elem;
+      const long long unit_bytes = batch_unit_bytes(bytes, grid_max_);
+      for (int k = 0; k < ndesc; k++) batch_regroup(use[k], unit_bytes);
+      // One launch per kernel family, in family order on the slot's stream.
+      BatchLayout L;
+      batch_layout(use, L, nullptr);
+      bool fits = true;
+      for (int f = 0; f < kBatchFamilies; f++) fits = fits && L.total[f] < (1LL << 31);
+      for (int k = 0; k < ndesc; k++) s.h_desc[k] = use[k];
+      for (size_t k = 0; k < L.start.size(); k++) s.h_start[k] = L.start[k];
+      ok = fits &&
+           hipMemcpyAsync(s.d_desc, s.h_desc, ndesc * sizeof(BatchDesc), hipMemcpyHostToDevice, s.stream) ==
+               hipSuccess &&
+           hipMemcpyAsync(s.d_start, s.h_start, L.start.size() * sizeof(int), hipMemcpyHostToDevice, s.stream) ==
+               hipSuccess &&
+           launch_batch_all(s.d_desc, s.d_start, L, grid_, s.stream) == hipSuccess &&
+           hipEventRecord(s.done, s.stream) == hipSuccess;
+      if (!ok) (void)hipGetLastError();
     }
+
+    lock.lock();
+    for (auto it = back.rbegin(); it != back.rend(); ++it) {
+      (*it)->status = 0;
+      pending_.push_front(*it);
+    }
+    if (!ok) failed.insert(failed.end(), take.begin(), take.end());
+    for (auto& r : failed) {
+      r->status = -1;
+      failed_++;
+      live_.erase(r->id);
+    }
+    if (!failed.empty()) done_cv_.notify_all();
+    if (!ok || take.empty()) return !failed.empty();
     s.reqs = std::move(take);
     s.busy = true;
     batches_++;
@@ -471,7 +489,7 @@ class Server {
             progressed = true;
           }
         }
-        if (!s.busy && !pending_.empty()) progressed |= fill_and_launch(s);
+        if (!s.busy && !pending_.empty()) progressed |= fill_and_launch(s, lock);
         any_busy |= s.busy;
       }
       if (progressed) done_cv_.notify_all();
