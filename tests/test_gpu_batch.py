@@ -218,3 +218,33 @@ def test_config3_full_batch_checked_on_device(gpu_stream):
     finally:
         A.free()
         B.free()
+
+
+def test_many_tiny_problems_between_large_ones(gpu_stream):
+    # Thousands of problems of a few elements between a few large ones: a
+    # workgroup that strides over units by the grid then skips many problems
+    # per unit, which the kernel's wave-parallel search (64 problems per
+    # step) must get right at every boundary, including runs longer than 64
+    # problems and the last problem of each family.
+    rng = np.random.default_rng(77)
+    probs = []
+    for i in range(3000):
+        if i % 700 == 350:
+            probs.append(W.random_problem(rng, 3, 4, 200000, 400000, [8]))
+        else:
+            r = int(rng.integers(2, 5))
+            d = [int(x) for x in rng.integers(1, 4, size=r)]
+            p = [int(x) for x in rng.permutation(r)]
+            probs.append((d, p, int(rng.choice([4, 8, 16]))))
+    say("case: 3000 problems, %d of them large" % sum(1 for d, p, e in probs if np.prod(d) > 1000))
+    slab = Slab(probs, seed=5)
+    try:
+        b = L.Batch(probs, gpu_stream)
+        b.set_buffers(slab.in_ptrs(), slab.out_ptrs())
+        assert b.execute(use_graph=False) == L.LIBRETT_SUCCESS
+        hip.stream_sync(gpu_stream)
+        bad = slab.check()
+        assert not bad, [probs[i] for i in bad[:5]]
+        b.destroy()
+    finally:
+        slab.free()
