@@ -450,6 +450,15 @@ def bench_selector(args, stream):
     pure, _ = S.evaluate(recs, sc)
     ss, sa, sp = S.summary(sel), S.summary(ana), S.summary(pure)
     ncand = sum(len(r["times"]) for r in recs)
+    # The pairs that decide each slowdown, timed again head to head.
+    t2 = time.perf_counter()
+    pk = S.picks(recs, sc, margin)
+    rs = S.summary(S.retime(recs, [(k, i, b) for k, i, m, b in pk], stream, A.ptr, B.ptr))
+    ra = S.summary(S.retime(recs, [(k, m, b) for k, i, m, b in pk if m >= 0], stream, A.ptr, B.ptr))
+    log("re-timed head to head (15 runs each): librettPlan mean %.4f (p50 %.3f p90 %.3f p99 %.3f max %.3f, %.1f%% "
+        "within 5%%); analytic %.4f (p90 %.3f p99 %.3f); %.0f s"
+        % (rs["mean"], rs["p50"], rs["p90"], rs["p99"], rs["max"], 100 * rs["within_5pct"], ra["mean"], ra["p90"],
+           ra["p99"], time.perf_counter() - t2))
     log("librettPlan (selector, margin %.2f): mean slowdown %.4f (p50 %.3f p90 %.3f p99 %.3f, %.1f%% within 5%%); "
         "selector alone %.4f (p90 %.3f, %.1f%% within 5%%); analytic %.4f (p50 %.3f p90 %.3f p99 %.3f, %.1f%% within "
         "5%%); %d problems, %d candidates timed in %.0f s; inference %.1f ms"
@@ -464,6 +473,7 @@ def bench_selector(args, stream):
         "config": {"workload": "learned selector: %d held-out problems, rank 2-7, 1 MB-512 MB, f32/f64/c128"
                                % len(recs), "seed": 1001, "candidates_timed": ncand},
         "selector": ss, "selector_margin": margin, "selector_alone": sp, "analytic": sa,
+        "retimed": {"librettPlan": rs, "analytic": ra, "runs_each": 15},
         "inference_ms_all_candidates": round(infer * 1e3, 2),
         "roofline": None, "cpu_baseline": None,
     }
@@ -475,7 +485,8 @@ def bench_selector(args, stream):
 def summarize_selector(r):
     return {"problems": r["config"]["workload"], "candidates_timed": r["config"]["candidates_timed"],
             "librettPlan_slowdown": r["selector"], "selector_alone_slowdown": r["selector_alone"],
-            "analytic_slowdown": r["analytic"], "margin": r["selector_margin"]}
+            "analytic_slowdown": r["analytic"], "margin": r["selector_margin"],
+            "retimed_head_to_head": r["retimed"]}
 
 
 def batched_problems(n, seed=20261202):

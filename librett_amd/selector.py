@@ -322,26 +322,83 @@ def scores(params, mean, std, recs, stream):
     return res
 
 
-def evaluate(recs, sc, margin=0.0):
-    """Slowdown of the selector's and the analytic model's choice versus the
-    fastest measured candidate, per problem. With a margin, the selector's
-    pick replaces the analytic choice only when its score is lower by more
-    than the margin (librettPlan's rule)."""
-    sel, ana = [], []
-    for r, s in zip(recs, sc):
+def picks(recs, sc, margin=0.0):
+    """Per problem with at least one timed candidate: (record index, the
+    selector's pick, the analytic model's pick, the fastest timed candidate).
+    With a margin, the selector's pick replaces the analytic choice only when
+    its score is lower by more than the margin (librettPlan's rule). The
+    analytic pick is -1 where the model's choice did not run."""
+    out = []
+    for k, (r, s) in enumerate(zip(recs, sc)):
         t = np.asarray(r["times"], dtype=np.float64)
         ok = t > 0
         if not ok.any():
             continue
-        best = t[ok].min()
+        b = int(np.argmin(np.where(ok, t, np.inf)))
         s = np.where(ok, np.asarray(s, dtype=np.float64), np.inf)
         i = int(np.argmin(s))
         m = r["model_choice"]
         if margin > 0 and 0 <= m < len(t) and s[m] - s[i] < margin:
             i = m
-        sel.append(t[i] / best)
-        ana.append(t[m] / best if 0 <= m < len(t) and t[m] > 0 else np.nan)
+        out.append((k, i, m if 0 <= m < len(t) and t[m] > 0 else -1, b))
+    return out
+
+
+def evaluate(recs, sc, margin=0.0):
+    """Slowdown of the selector's and the analytic model's choice versus the
+    fastest measured candidate, per problem (margin as in picks())."""
+    sel, ana = [], []
+    for k, i, m, b in picks(recs, sc, margin):
+        t = np.asarray(recs[k]["times"], dtype=np.float64)
+        sel.append(t[i] / t[b])
+        ana.append(t[m] / t[b] if m >= 0 else np.nan)
     return np.array(sel), np.array(ana)
+
+
+def retime(recs, chosen, stream, in_ptr, out_ptr, rounds=15):
+    """Slowdowns of chosen candidates, timed again head to head with the
+    fastest one. chosen: (record index, pick, best) triples. A pick that is
+    the best counts 1.0. Otherwise both are planned (librett_c_plan_candidate)
+    and run alternately, `rounds` times each, every run after an output
+    memset as in librettPlanMeasure's loop, with the first of the pair
+    alternating between rounds; the slowdown is the ratio of the medians.
+    One timing per candidate (the exhaustive sweep) is noisy on problems
+    that run for a few microseconds; this pass measures the pairs that
+    decide the slowdown with many runs."""
+    from . import hip, librettDestroy, librettExecute, plan_candidate, LIBRETT_SUCCESS
+    e0, e1 = hip.Event(), hip.Event()
+    res = []
+    for k, i, b in chosen:
+        if i == b:
+            res.append(1.0)
+            continue
+        r = recs[k]
+        d, p, e = r["dims"], r["perm"], r["elem"]
+        n = int(np.prod(d)) * e
+        hs = []
+        for idx in (i, b):
+            rc, h = plan_candidate(len(d), d, p, e, stream, idx)
+            if rc != LIBRETT_SUCCESS:
+                raise RuntimeError("plan_candidate %d of %s %s failed: %d" % (idx, d, p, rc))
+            hs.append(h)
+        ts = ([], [])
+        for h in hs:
+            librettExecute(h, in_ptr, out_ptr)
+        hip.stream_sync(stream)
+        for rd in range(rounds):
+            for j in ((0, 1) if rd % 2 == 0 else (1, 0)):
+                hip.memset(out_ptr, 0xFF, n, stream)
+                e0.record(stream)
+                librettExecute(hs[j], in_ptr, out_ptr)
+                e1.record(stream)
+                e1.synchronize()
+                ts[j].append(e1.elapsed_ms(e0))
+        for h in hs:
+            librettDestroy(h)
+        res.append(float(np.median(ts[0]) / np.median(ts[1])))
+    e0.destroy()
+    e1.destroy()
+    return np.array(res)
 
 
 def summary(slow):

@@ -214,3 +214,27 @@ def test_plan_follows_ensemble_mean(gpu_stream):
         L.librettDestroy(h)
     finally:
         L.selector_set(None, None, None)
+
+
+def test_retime_pairs(gpu_stream):
+    # A 64 MiB f32 transpose: long enough that head-to-head timings agree.
+    dim, perm, elem = [64, 1024, 256], [2, 0, 1], 4
+    say("case: head-to-head re-timing", dim, perm, elem)
+    n = int(np.prod(dim)) * elem
+    a, b = hip.DeviceBuffer(n), hip.DeviceBuffer(n)
+    hip.memset(a.ptr, 0, n)
+    t = L.measure_candidates(dim, perm, elem, gpu_stream, a.ptr, b.ptr)
+    ran = [i for i in range(len(t)) if t[i] > 0]
+    assert len(ran) >= 2
+    best = min(ran, key=lambda i: t[i])
+    worst = max(ran, key=lambda i: t[i])
+    rec = {"dims": dim, "perm": perm, "elem": elem, "times": t}
+    r = S.retime([rec], [(0, best, best), (0, worst, best), (0, best, worst)], gpu_stream, a.ptr, b.ptr, rounds=7)
+    assert r[0] == 1.0
+    assert np.all(np.isfinite(r)) and np.all(r > 0)
+    # The two orders of one pair are reciprocal up to timing noise, and the
+    # slow candidate stays slower than the fast one when the sweep saw a
+    # clear gap.
+    assert abs(r[1] * r[2] - 1) < 0.2, r
+    if t[worst] > 1.5 * t[best]:
+        assert r[1] > 1.2, (r, t[worst], t[best])

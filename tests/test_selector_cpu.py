@@ -265,6 +265,21 @@ def test_margin_rule_in_evaluation():
     assert list(sel) == [1.0]
 
 
+def test_picks_name_the_pairs_evaluate_divides():
+    recs = [{"times": np.array([1.0, 1.2, 0.9]), "model_choice": 1},
+            {"times": np.array([-1.0, 2.0, 3.0]), "model_choice": 0},
+            {"times": np.array([-1.0, -1.0]), "model_choice": 0}]
+    sc = [np.array([0.0, 0.05, 0.2]), np.array([-9.0, 0.3, 0.1]), np.array([0.0, 0.0])]
+    # Record 0: selector argmin 0, analytic 1, best 2; record 1: candidate 0
+    # did not run (never picked, analytic -1), argmin 2, best 1; record 2:
+    # nothing ran, skipped.
+    assert S.picks(recs, sc) == [(0, 0, 1, 2), (1, 2, -1, 1)]
+    # Margin 0.1: record 0 keeps the analytic choice (0.05 < 0.1).
+    assert S.picks(recs, sc, margin=0.1) == [(0, 1, 1, 2), (1, 2, -1, 1)]
+    sel, ana = S.evaluate(recs, sc, margin=0.1)
+    assert np.allclose(sel, [1.2 / 0.9, 1.5]) and ana[0] == 1.2 / 0.9 and np.isnan(ana[1])
+
+
 def test_margin_default_and_override():
     code = "import librett_amd as L; print(L.selector_margin())"
     env = dict(os.environ)
