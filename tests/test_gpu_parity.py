@@ -109,6 +109,8 @@ def test_every_candidate_small_shapes(gpu_stream, bufs, elem):
         ([34, 18, 5], [1, 0, 2]),   # f32: 8-byte lanes both ways
         ([130, 46], [1, 0]),        # f32: 8-byte lanes, partial tiles
         ([68, 50, 3], [1, 0, 2]),   # f32: 16-byte input lanes, 8-byte output lanes
+        ([12, 2, 2, 5, 33, 6], [4, 2, 1, 5, 0, 3]),  # planes with y over the output's leading run
+        ([5, 14, 6, 6, 9, 3], [4, 2, 1, 5, 0, 3]),
     ]
     for dim, perm in shapes:
         n = len(L.candidates(dim, perm, elem))

@@ -109,6 +109,19 @@ def test_f32_eight_byte_lanes():
     assert {(2, 2), (4, 2), (2, 4), (2, 1), (1, 2)} <= seen
 
 
+def test_output_span_planes():
+    # Planes whose y group is the output's whole leading run that holds no
+    # x-group dim: y = 4 dims for (288,2,2,16,589,6) (the output's leading
+    # dims 589, 2, 2, 6 before input dim 0), y = 2 dims with x = 2 dims for
+    # (5,44,6,6,207,41); each replays bit-exact, scaled down.
+    for dim, perm, want in (([12, 2, 2, 5, 33, 6], [4, 2, 1, 5, 0, 3], "groups 1/4"),
+                            ([5, 14, 6, 6, 9, 3], [4, 2, 1, 5, 0, 3], "groups 2/2"),
+                            ([11, 13, 7], [2, 1, 0], "groups 1/2")):
+        for elem in (4, 8):
+            descs = [c["desc"] for c in replay_all(dim, perm, elem)]
+            assert any(want in d for d in descs), (dim, elem, want)
+
+
 def test_bench7_high_rank_scaled():
     # tests/librett_bench.cpp:718,738 shapes scaled down for the host replay.
     replay_all([5, 3, 2, 4, 3, 3, 3, 4], [7, 2, 5, 0, 1, 3, 6, 4], 8)
