@@ -405,7 +405,7 @@ def summary(slow):
     slow = slow[np.isfinite(slow)]
     return {"mean": float(slow.mean()), "geomean": float(np.exp(np.log(slow).mean())),
             "p50": float(np.median(slow)), "p90": float(np.percentile(slow, 90)),
-            "p99": float(np.percentile(slow, 99)), "max": float(slow.max()),
+            "p95": float(np.percentile(slow, 95)), "p99": float(np.percentile(slow, 99)), "max": float(slow.max()),
             "within_5pct": float((slow <= 1.05).mean()), "exact": float((slow <= 1.0 + 1e-9).mean()),
             "problems": int(len(slow))}
 
