@@ -30,6 +30,7 @@
 #include <memory>
 #include <mutex>
 #include <random>
+#include <set>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -59,28 +60,27 @@ class Arena {
       base_ = nullptr;
       return false;
     }
-    free_[0] = cap_;
+    add_free(0, cap_);
     return true;
   }
   ~Arena() {
     if (base_) (void)dev_free(base_);
   }
+  // Best fit: the smallest free range that holds the request (the lowest
+  // such offset among equals), found in O(log n) under the lock.
   void* alloc(size_t bytes) {
     bytes = std::max<size_t>(256, (bytes + 255) / 256 * 256);
     std::lock_guard<std::mutex> lock(mu_);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= bytes) {
-        const size_t off = it->first;
-        const size_t rest = it->second - bytes;
-        free_.erase(it);
-        if (rest) free_[off + bytes] = rest;
-        used_[off] = bytes;
-        in_use_ += bytes;
-        peak_ = std::max(peak_, in_use_);
-        return (char*)base_ + off;
-      }
-    }
-    return nullptr;
+    auto it = by_size_.lower_bound({bytes, 0});
+    if (it == by_size_.end()) return nullptr;
+    const size_t size = it->first, off = it->second;
+    by_size_.erase(it);
+    by_off_.erase(off);
+    if (size > bytes) add_free(off + bytes, size - bytes);
+    used_[off] = bytes;
+    in_use_ += bytes;
+    peak_ = std::max(peak_, in_use_);
+    return (char*)base_ + off;
   }
   bool release(void* p) {
     std::lock_guard<std::mutex> lock(mu_);
@@ -92,20 +92,24 @@ class Arena {
     used_.erase(u);
     in_use_ -= size;
     size_t start = off;
-    auto next = free_.lower_bound(off);
-    if (next != free_.end() && next->first == off + size) {
+    // Merge with the free ranges on either side.
+    auto next = by_off_.find(off + size);
+    if (next != by_off_.end()) {
+      by_size_.erase({next->second, next->first});
       size += next->second;
-      next = free_.erase(next);
+      by_off_.erase(next);
     }
-    if (next != free_.begin()) {
-      auto prev = std::prev(next);
+    auto after = by_off_.lower_bound(off);
+    if (after != by_off_.begin()) {
+      auto prev = std::prev(after);
       if (prev->first + prev->second == start) {
+        by_size_.erase({prev->second, prev->first});
         start = prev->first;
         size += prev->second;
-        free_.erase(prev);
+        by_off_.erase(prev);
       }
     }
-    free_[start] = size;
+    add_free(start, size);
     return true;
   }
   size_t in_use() {
@@ -125,9 +129,14 @@ class Arena {
   bool owns(const void* p) const { return p >= base_ && p < (const char*)base_ + cap_; }
 
  private:
+  void add_free(size_t off, size_t size) {
+    by_off_[off] = size;
+    by_size_.insert({size, off});
+  }
   void* base_ = nullptr;
   size_t cap_ = 0, in_use_ = 0, peak_ = 0;
-  std::map<size_t, size_t> free_;
+  std::map<size_t, size_t> by_off_;                   // free ranges: offset -> size
+  std::set<std::pair<size_t, size_t>> by_size_;       // the same ranges as (size, offset)
   std::unordered_map<size_t, size_t> used_;
   std::mutex mu_;
 };
