@@ -232,9 +232,10 @@ def test_retime_pairs(gpu_stream):
     r = S.retime([rec], [(0, best, best), (0, worst, best), (0, best, worst)], gpu_stream, a.ptr, b.ptr, rounds=7)
     assert r[0] == 1.0
     assert np.all(np.isfinite(r)) and np.all(r > 0)
-    # The two orders of one pair are reciprocal up to timing noise, and the
-    # slow candidate stays slower than the fast one when the sweep saw a
+    # The two orders of one pair are reciprocal up to timing noise (the
+    # slowest candidate of a sweep can vary by a third between runs), and
+    # the slow candidate stays slower than the fast one when the sweep saw a
     # clear gap.
-    assert abs(r[1] * r[2] - 1) < 0.2, r
+    assert 1 / 1.5 < r[1] * r[2] < 1.5, r
     if t[worst] > 1.5 * t[best]:
-        assert r[1] > 1.2, (r, t[worst], t[best])
+        assert r[1] > 1.2 and r[2] < 1 / 1.2, (r, t[worst], t[best])
