@@ -59,6 +59,44 @@ def test_alone_equals_mix():
     srv.destroy()
 
 
+def test_client_threads_share_the_engine():
+    # Four host threads allocate from the arena, submit, wait and free at
+    # the same time (ctypes releases the GIL in the library's calls); every
+    # result equals the oracle and the arena ends empty.
+    import threading
+    say("case: serving from 4 client threads, 40 requests each")
+    srv = L.Server(1 << 30, slots=4, max_batch=16)
+    errors = []
+
+    def client(k):
+        try:
+            for i, (d, p, e) in enumerate(requests(40, 100 + k)):
+                x = data.uniform(int(np.prod(d)), e, 10000 * k + i)
+                pin, pout = srv.alloc(x.nbytes), srv.alloc(x.nbytes)
+                assert pin is not None and pout is not None
+                hip.htod(pin, x)
+                srv.wait(srv.submit(d, p, e, pin, pout))
+                y = np.empty_like(x)
+                hip.dtoh(y, pout)
+                if not np.array_equal(y.view(np.uint8), oracle.transpose(d, p, e, x).view(np.uint8)):
+                    errors.append((k, i, d, p, e))
+                assert srv.free(pin) == L.LIBRETT_SUCCESS and srv.free(pout) == L.LIBRETT_SUCCESS
+        except Exception as ex:  # reported below, from the main thread
+            errors.append((k, repr(ex)))
+
+    ts = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    st = srv.stats()
+    say("  stats", {k: st[k] for k in ("completed", "batches", "mean_batch", "failed")})
+    assert not errors, errors[:3]
+    assert st["completed"] == 160 and st["failed"] == 0
+    assert st["arena_in_use_bytes"] == 0
+    srv.destroy()
+
+
 def test_small_arena_admission_and_accounting():
     say("case: serving with a 64 MiB arena")
     budget = 64 << 20
