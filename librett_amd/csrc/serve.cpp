@@ -26,16 +26,15 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
-#include <map>
 #include <memory>
 #include <mutex>
 #include <random>
-#include <set>
 #include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
+#include "arena.h"
 #include "devmem.h"
 #include "../../include/librett_c.h"
 #include "batch.h"
@@ -51,94 +50,41 @@ double now_s() {
 
 // First-fit arena over one device allocation, 256-byte granules, with
 // coalescing of free neighbours.
+// The serving engine's device arena: one allocation from the device
+// allocator, handed out in ranges by a RangeAllocator.
 class Arena {
  public:
   bool init(size_t bytes) {
-    cap_ = (bytes + 255) / 256 * 256;
-    if (dev_malloc(&base_, cap_) != hipSuccess) {
+    const size_t cap = (bytes + RangeAllocator::kGranule - 1) / RangeAllocator::kGranule * RangeAllocator::kGranule;
+    if (dev_malloc(&base_, cap) != hipSuccess) {
       (void)hipGetLastError();
       base_ = nullptr;
       return false;
     }
-    add_free(0, cap_);
+    ranges_.reset(cap);
     return true;
   }
   ~Arena() {
     if (base_) (void)dev_free(base_);
   }
-  // Best fit: the smallest free range that holds the request (the lowest
-  // such offset among equals), found in O(log n) under the lock.
   void* alloc(size_t bytes) {
-    bytes = std::max<size_t>(256, (bytes + 255) / 256 * 256);
-    std::lock_guard<std::mutex> lock(mu_);
-    auto it = by_size_.lower_bound({bytes, 0});
-    if (it == by_size_.end()) return nullptr;
-    const size_t size = it->first, off = it->second;
-    by_size_.erase(it);
-    by_off_.erase(off);
-    if (size > bytes) add_free(off + bytes, size - bytes);
-    used_[off] = bytes;
-    in_use_ += bytes;
-    peak_ = std::max(peak_, in_use_);
-    return (char*)base_ + off;
+    const size_t off = ranges_.alloc(bytes);
+    return off == RangeAllocator::kNone ? nullptr : (char*)base_ + off;
   }
   bool release(void* p) {
-    std::lock_guard<std::mutex> lock(mu_);
-    if (!p || p < base_) return false;
-    const size_t off = (size_t)((char*)p - (char*)base_);
-    auto u = used_.find(off);
-    if (u == used_.end()) return false;
-    size_t size = u->second;
-    used_.erase(u);
-    in_use_ -= size;
-    size_t start = off;
-    // Merge with the free ranges on either side.
-    auto next = by_off_.find(off + size);
-    if (next != by_off_.end()) {
-      by_size_.erase({next->second, next->first});
-      size += next->second;
-      by_off_.erase(next);
-    }
-    auto after = by_off_.lower_bound(off);
-    if (after != by_off_.begin()) {
-      auto prev = std::prev(after);
-      if (prev->first + prev->second == start) {
-        by_size_.erase({prev->second, prev->first});
-        start = prev->first;
-        size += prev->second;
-        by_off_.erase(prev);
-      }
-    }
-    add_free(start, size);
-    return true;
+    if (!p || !owns(p)) return false;
+    return ranges_.release((size_t)((char*)p - (char*)base_));
   }
-  size_t in_use() {
-    std::lock_guard<std::mutex> lock(mu_);
-    return in_use_;
-  }
-  size_t peak() {
-    std::lock_guard<std::mutex> lock(mu_);
-    return peak_;
-  }
+  size_t in_use() { return ranges_.in_use(); }
+  size_t peak() { return ranges_.peak(); }
   // Restarts the peak at the bytes in use now.
-  void reset_peak() {
-    std::lock_guard<std::mutex> lock(mu_);
-    peak_ = in_use_;
-  }
-  size_t capacity() const { return cap_; }
-  bool owns(const void* p) const { return p >= base_ && p < (const char*)base_ + cap_; }
+  void reset_peak() { ranges_.reset_peak(); }
+  size_t capacity() const { return ranges_.capacity(); }
+  bool owns(const void* p) const { return p >= base_ && p < (const char*)base_ + ranges_.capacity(); }
 
  private:
-  void add_free(size_t off, size_t size) {
-    by_off_[off] = size;
-    by_size_.insert({size, off});
-  }
   void* base_ = nullptr;
-  size_t cap_ = 0, in_use_ = 0, peak_ = 0;
-  std::map<size_t, size_t> by_off_;                   // free ranges: offset -> size
-  std::set<std::pair<size_t, size_t>> by_size_;       // the same ranges as (size, offset)
-  std::unordered_map<size_t, size_t> used_;
-  std::mutex mu_;
+  RangeAllocator ranges_;
 };
 
 struct Request {
