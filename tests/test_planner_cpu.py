@@ -122,6 +122,76 @@ def test_output_span_planes():
             assert any(want in d for d in descs), (dim, elem, want)
 
 
+def expected_planes(dim, perm):
+    """(x dims, y dims) of the Tiled planes the planner builds for a
+    reduced problem (planner.cpp add_tiled): the reference's plane (1, 1);
+    x the leading input dims reaching 64 elements and y the same on the
+    output side, where dim 0 (or output dim 0) is short; and y the output's
+    whole leading run that holds no x dim. Groups never share a dim."""
+    n = len(dim)
+
+    def reach(inp):
+        v, k = 1, 0
+        while k < n and k < 6:
+            v *= dim[k if inp else perm[k]]
+            k += 1
+            if v >= 64:
+                break
+        return k
+
+    def span(a):
+        v, k = 1, 0
+        while k < n and k < 6:
+            d = perm[k]
+            if d < a or v * dim[d] > (1 << 30):
+                break
+            v *= dim[d]
+            k += 1
+        return k
+
+    a64, b64 = reach(True), reach(False)
+    combos = [(a64, 1), (1, b64), (a64, b64), (1, span(1)), (a64, span(a64))]
+    out = {(1, 1)}
+    for i, (a, b) in enumerate(combos):
+        if a < 1 or b < 1 or (a, b) == (1, 1):
+            continue
+        if i < 3 and ((a > 1 and dim[0] >= 64) or (b > 1 and dim[perm[0]] >= 64)):
+            continue
+        if set(range(a)) & set(perm[:b]):
+            continue
+        out.add((a, b))
+    return out
+
+
+def test_tiled_planes_follow_the_rules():
+    # Every plane the rules name appears among the candidates (some tile
+    # shape of it), and no other, on random problems at 4 and 8 bytes.
+    rng = np.random.default_rng(11)
+    checked = 0
+    for _ in range(300):
+        r = int(rng.integers(2, 7))
+        dim = [int(v) for v in rng.integers(2, 40, r)]
+        perm = [int(v) for v in rng.permutation(r)]
+        rd, rp = L.reduce_ranks(dim, perm)
+        if len(rd) < 2 or rp[0] == 0:
+            continue
+        want = expected_planes(rd, rp)
+        for elem in (4, 8):
+            got = set()
+            for c in L.host_candidates(dim, perm, elem):
+                if c["method"] != 4:
+                    continue
+                d = c["desc"]
+                if " groups " in d:
+                    a, b = d.split(" groups ")[1].split(" ")[0].split("/")
+                    got.add((int(a), int(b)))
+                else:
+                    got.add((1, 1))
+            assert got == want, (dim, perm, rd, rp, elem, got, want)
+            checked += 1
+    assert checked > 200
+
+
 def test_bench7_high_rank_scaled():
     # tests/librett_bench.cpp:718,738 shapes scaled down for the host replay.
     replay_all([5, 3, 2, 4, 3, 3, 3, 4], [7, 2, 5, 0, 1, 3, 6, 4], 8)
